@@ -1,0 +1,169 @@
+"""Throughput benchmark of the FlowLB core on one MI355X.
+
+Default workload: the 8192 x 2048 free-surface culvert (dam break behind an
+embankment with a 240-cell barrel, gravity 2e-5, tau = 0.51), FP32 population
+storage.  A step is one full free-surface lattice step (passes 1-4 around
+stream-collide) over the whole grid, inputs resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--storage fp32]
+
+Prints one JSON line: MLUPS over all cells, the roofline of the dominant
+kernel (stream-collide) from HIP-event timing of its launches, the FP16/BF16
+storage variants and the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MLUPS & % of HBM roofline, FP32 vs FP16-store; adjoint s/gradient"
+HBM_PEAK_GBS = 8000.0
+STORAGE_BYTES = {"fp32": 4, "fp16s": 2, "bf16s": 2}
+
+
+def active_counts(solver):
+    """(active, inactive) cells from the device flag histogram: WALL and any
+    GAS-flagged cell are inactive."""
+    h = solver.flag_histogram()
+    vals = np.arange(256)
+    inactive = h[(vals & 0x0A) != 0].sum()
+    return int(h.sum() - inactive), int(inactive)
+
+
+def algorithmic_bytes(active, inactive, s, free_surface):
+    """Per-launch bytes of stream-collide and per-step bytes of the whole step.
+    Stream-collide: 18 s (nine loads, nine stores) + 1 flag byte per active cell,
+    + 12 (rho, u stores read by the free-surface passes) in free-surface runs;
+    1 flag byte per inactive cell.  Free-surface step: 18 s + 38 per active cell."""
+    sc = active * (18 * s + 1 + (12 if free_surface else 0)) + inactive
+    step = active * (18 * s + (38 if free_surface else 1)) + inactive
+    return sc, step
+
+
+def cpu_baseline(case, seconds=15.0):
+    """FP64 serial oracle on the full grid of the workload, a few steps."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    flags, rho0, u0 = case.build()
+    o = pyoracle.Oracle(flags, case.nx, case.ny, case.tau, rho0=rho0.astype(np.float64),
+                        u0=u0.astype(np.float64), use_force=case.force != (0.0, 0.0),
+                        fx=case.force[0], fy=case.force[1], free_surface=case.free_surface,
+                        mass_mode=1, threads=1)
+    o.step(1)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.step(1)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt > seconds or n >= 1000:
+            break
+    o.close()
+    return dict(value=round(case.N * n / dt / 1e6, 3), unit="MLUPS", cores=1, kind="port",
+                sample=f"FP64 serial oracle, {n} steps of the full {case.nx}x{case.ny} grid "
+                       f"after 1 warm-up step ({dt:.1f} s)")
+
+
+def measure(case, storage, steps, warmup, profile_steps, **opt):
+    s = case.solver(storage, **opt)
+    try:
+        s.step(warmup)
+        s.sync()
+        a0, i0 = active_counts(s)
+        t0 = time.perf_counter()
+        ms = s.step_timed(steps)
+        wall = time.perf_counter() - t0
+        a1, i1 = active_counts(s)
+        active, inactive = (a0 + a1) / 2, (i0 + i1) / 2
+        tot_k, per_k = s.step_timed(profile_steps, per_kernel=True)
+        mass_ok = np.isfinite(s.mass())
+    finally:
+        s.close()
+    sbytes = STORAGE_BYTES[storage]
+    sc_b, step_b = algorithmic_bytes(active, inactive, sbytes, case.free_surface)
+    sc_ms = per_k[0] / profile_steps
+    return dict(ms=ms, wall=wall, active=active, inactive=inactive, sc_bytes=sc_b,
+                step_bytes=step_b, sc_ms=sc_ms, per_kernel_ms=[v / profile_steps for v in per_k],
+                finite=bool(mass_ok))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="free_surface_8192x2048")
+    ap.add_argument("--storage", default="fp32", choices=list(STORAGE_BYTES))
+    ap.add_argument("--profile-steps", type=int, default=30)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the workload is one domain on one MI355X; "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from flowlb_amd import cases
+    case = cases.CASES[args.workload]
+    r = measure(case, args.storage, args.steps, args.warmup, args.profile_steps)
+    ms_step = r["ms"] / args.steps
+    mlups = case.N / (ms_step * 1e-3) / 1e6
+    achieved = r["sc_bytes"] / (r["sc_ms"] * 1e-3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_{args.workload}_{args.storage}.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.storage == "fp32" else f"fp32 arithmetic, {args.storage} storage",
+        "data": "synthetic",
+        "config": {"workload": case.name, "nx": case.nx, "ny": case.ny, "storage": args.storage,
+                   "free_surface": case.free_surface, "tau": case.tau,
+                   "force": list(case.force), "mass_mode": "conservative",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "stream_collide",
+                     "algorithmic_bytes_per_launch": int(r["sc_bytes"]),
+                     "avg_launch_ms": round(r["sc_ms"], 5)},
+        "mlups_active_cells": round(r["active"] / (ms_step * 1e-3) / 1e6, 1),
+        "step_roofline": {"algorithmic_bytes_per_step": int(r["step_bytes"]),
+                          "achieved_GBs": round(r["step_bytes"] / (ms_step * 1e-3) / 1e9, 1),
+                          "frac": round(r["step_bytes"] / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernel_ms": {k: round(v, 5) for k, v in zip(
+            ["stream_collide", "fs1", "fs2", "fs3", "fs4"], r["per_kernel_ms"])},
+        "wall_s_timed_region": round(r["wall"], 4),
+    }
+    if not args.no_sweep:
+        sweep = {}
+        for st in STORAGE_BYTES:
+            if st == args.storage:
+                sweep[st] = out["value"]
+                continue
+            q = measure(case, st, max(50, args.steps // 2), args.warmup // 2, 10)
+            sweep[st] = round(case.N / (q["ms"] / max(50, args.steps // 2) * 1e-3) / 1e6, 1)
+        out["storage_mlups"] = sweep
+        out["fp16s_speedup"] = round(sweep["fp16s"] / sweep["fp32"], 3)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

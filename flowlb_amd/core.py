@@ -1,0 +1,318 @@
+"""ctypes binding of libflbcore.so (the C ABI in include/flb_core.h).
+
+The library must be built (python -m flowlb_amd.build); there is no fallback
+implementation: loading fails loudly when the shared object is missing.
+"""
+import ctypes as C  # noqa: N812
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libflbcore.so"
+
+FLUID, GAS, INTERFACE, WALL, MOVING_WALL, INLET, OUTLET, GAS_INTERFACE = (
+    0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80)
+STORAGE = {"fp32": 0, "fp16s": 1, "bf16s": 2}
+K_STREAM_COLLIDE, K_FS1, K_FS2, K_FS3, K_FS4 = range(5)
+KERNEL_NAMES = ["stream_collide", "fs1", "fs2", "fs3", "fs4"]
+
+
+class FlbError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+class Options(C.Structure):
+    _fields_ = [("tau", C.c_float), ("fx", C.c_float), ("fy", C.c_float),
+                ("use_force", C.c_int32), ("collision", C.c_int32), ("smagorinsky", C.c_int32),
+                ("problem", C.c_int32), ("storage", C.c_int32),
+                ("gas_check_corrected", C.c_int32), ("mass_mode", C.c_int32),
+                ("block_x", C.c_int32), ("device", C.c_int32)]
+
+
+class Rect(C.Structure):
+    _fields_ = [("x0", C.c_double), ("y0", C.c_double), ("height", C.c_double),
+                ("width", C.c_double)]
+
+
+class Channel(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("dx", C.c_double),
+                ("x_min", C.c_double), ("y_min", C.c_double), ("free_surface", C.c_int32),
+                ("bc_left", C.c_uint8), ("bc_right", C.c_uint8), ("bc_up", C.c_uint8),
+                ("bc_down", C.c_uint8), ("n_obstacles", C.c_uint32), ("n_initial", C.c_uint32),
+                ("obstacles", C.POINTER(Rect)), ("initial", C.POINTER(Rect))]
+
+
+class Culvert(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("free_surface", C.c_int32),
+                ("bc_left", C.c_uint8), ("bc_right", C.c_uint8), ("bc_up", C.c_uint8),
+                ("bc_down", C.c_uint8), ("emb_x0", C.c_uint32), ("emb_x1", C.c_uint32),
+                ("opening", C.c_uint32), ("water_x1", C.c_uint32), ("water_depth", C.c_uint32),
+                ("u_inlet", C.c_float), ("rho_noise", C.c_double), ("seed", C.c_uint64)]
+
+
+class UnitsIn(C.Structure):
+    _fields_ = [("length_si", C.c_double), ("nx", C.c_uint32), ("ref_velocity_si", C.c_double),
+                ("ref_velocity_lb", C.c_double), ("density_si", C.c_double),
+                ("relaxation_time", C.c_double), ("kinematic_viscosity", C.c_double),
+                ("use_force", C.c_int32), ("acceleration_x", C.c_double),
+                ("acceleration_y", C.c_double), ("volumetric_force_x", C.c_double),
+                ("volumetric_force_y", C.c_double), ("velocity_si_x", C.c_double),
+                ("velocity_si_y", C.c_double)]
+
+
+class UnitsOut(C.Structure):
+    _fields_ = [("tau", C.c_float), ("inv_tau", C.c_float), ("fx", C.c_float), ("fy", C.c_float),
+                ("velocity_lb_x", C.c_double), ("velocity_lb_y", C.c_double),
+                ("length_scale", C.c_double), ("time_scale", C.c_double),
+                ("mass_scale", C.c_double)]
+
+
+# Exported symbols and their prototypes: (name, restype, argtypes).
+_H = C.c_void_p
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_U8 = C.POINTER(C.c_uint8)
+_U64 = C.POINTER(C.c_uint64)
+PROTOTYPES = [
+    ("flb_default_options", None, [C.POINTER(Options)]),
+    ("flb_create", C.c_int, [C.POINTER(Options), _U8, C.c_uint32, C.c_uint32, _F, _F,
+                             C.POINTER(_H)]),
+    ("flb_set_fs_fields", C.c_int, [_H, _F, _F, _F]),
+    ("flb_step", C.c_int, [_H, C.c_uint64]),
+    ("flb_sync", C.c_int, [_H]),
+    ("flb_read", C.c_int, [_H, _F, _F, _U8, _F]),
+    ("flb_read_fs", C.c_int, [_H, _F, _F]),
+    ("flb_read_populations", C.c_int, [_H, _F]),
+    ("flb_write_populations", C.c_int, [_H, _F]),
+    ("flb_device_fields", C.c_int, [_H, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    ("flb_mass", C.c_int, [_H, _D]),
+    ("flb_time", C.c_uint64, [_H]),
+    ("flb_step_timed", C.c_int, [_H, C.c_uint64, _F, _F]),
+    ("flb_flag_histogram", C.c_int, [_H, _U64]),
+    ("flb_last_error", C.c_char_p, [_H]),
+    ("flb_status_string", C.c_char_p, [C.c_int]),
+    ("flb_destroy", None, [_H]),
+    ("flb_build_channel_flags", C.c_int, [C.POINTER(Channel), _U8]),
+    ("flb_seed_free_surface", C.c_int, [_U8, C.c_uint32, C.c_uint32]),
+    ("flb_init_free_surface_fields", C.c_int, [_U8, C.c_size_t, _F, _F, _F]),
+    ("flb_build_culvert", C.c_int, [C.POINTER(Culvert), _U8, _F, _F]),
+    ("flb_units", C.c_int, [C.POINTER(UnitsIn), C.POINTER(UnitsOut)]),
+]
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m flowlb_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in PROTOTYPES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _register_optional(L)
+        _lib = L
+    return _lib
+
+
+_optional_hooks = []
+
+
+def _register_optional(L):
+    for hook in _optional_hooks:
+        hook(L)
+
+
+def fptr(a):
+    return None if a is None else a.ctypes.data_as(_F)
+
+
+def u8ptr(a):
+    return None if a is None else a.ctypes.data_as(_U8)
+
+
+def _f32(a, n=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def check(status, h=None, what=""):
+    if status != 0:
+        msg = lib().flb_last_error(h).decode() if h else ""
+        name = lib().flb_status_string(status).decode()
+        raise FlbError(status, f"{what}: {name}{': ' + msg if msg else ''}")
+
+
+def default_options(**kw):
+    o = Options()
+    lib().flb_default_options(C.byref(o))
+    for k, v in kw.items():
+        if k == "storage" and isinstance(v, str):
+            v = STORAGE[v]
+        setattr(o, k, v)
+    return o
+
+
+class Solver:
+    """One simulation on the GPU: the role h_launchCudaCalculations2D plays in
+    the reference (src/cuda/runCuda.cu:31-128), split into calls."""
+
+    def __init__(self, flags, nx, ny, options=None, rho0=None, u0=None, **kw):
+        N = nx * ny
+        self.nx, self.ny, self.N = nx, ny, N
+        self.options = options if options is not None else default_options(**kw)
+        self.flags0 = np.ascontiguousarray(flags, dtype=np.uint8)
+        if self.flags0.size != N:
+            raise ValueError("flags must have nx*ny entries")
+        rho0 = _f32(rho0, N)
+        u0 = _f32(u0, 2 * N)
+        h = C.c_void_p()
+        st = lib().flb_create(C.byref(self.options), u8ptr(self.flags0), nx, ny, fptr(rho0),
+                              fptr(u0), C.byref(h))
+        check(st, None, "flb_create")
+        self.h = h
+        self.free_surface = self.options.problem == 1
+
+    # -- set-up
+    def set_fs_fields(self, mass, phi, excess=None):
+        mass, phi, excess = _f32(mass, self.N), _f32(phi, self.N), _f32(excess, self.N)
+        check(lib().flb_set_fs_fields(self.h, fptr(mass), fptr(phi), fptr(excess)), self.h,
+              "flb_set_fs_fields")
+
+    # -- advance
+    def step(self, n=1):
+        check(lib().flb_step(self.h, n), self.h, "flb_step")
+
+    def sync(self):
+        check(lib().flb_sync(self.h), self.h, "flb_sync")
+
+    def step_timed(self, n, per_kernel=False):
+        tot = C.c_float()
+        if per_kernel:
+            arr = (C.c_float * 5)()
+            check(lib().flb_step_timed(self.h, n, C.byref(tot), arr), self.h, "flb_step_timed")
+            return tot.value, list(arr)
+        check(lib().flb_step_timed(self.h, n, C.byref(tot), None), self.h, "flb_step_timed")
+        return tot.value
+
+    @property
+    def time(self):
+        return lib().flb_time(self.h)
+
+    # -- readback
+    def read(self):
+        N = self.N
+        u = np.empty(2 * N, np.float32)
+        rho = np.empty(N, np.float32)
+        fl = np.empty(N, np.uint8)
+        phi = np.empty(N, np.float32) if self.free_surface else None
+        check(lib().flb_read(self.h, fptr(u), fptr(rho), u8ptr(fl), fptr(phi)), self.h, "flb_read")
+        out = dict(u=u, rho=rho, flags=fl)
+        if phi is not None:
+            out["phi"] = phi
+            m = np.empty(N, np.float32)
+            e = np.empty(N, np.float32)
+            check(lib().flb_read_fs(self.h, fptr(m), fptr(e)), self.h, "flb_read_fs")
+            out["mass"], out["excess"] = m, e
+        return out
+
+    def populations(self):
+        f = np.empty(9 * self.N, np.float32)
+        check(lib().flb_read_populations(self.h, fptr(f)), self.h, "flb_read_populations")
+        return f
+
+    def write_populations(self, f):
+        f = _f32(f, 9 * self.N)
+        check(lib().flb_write_populations(self.h, fptr(f)), self.h, "flb_write_populations")
+
+    def mass(self):
+        m = C.c_double()
+        check(lib().flb_mass(self.h, C.byref(m)), self.h, "flb_mass")
+        return m.value
+
+    def flag_histogram(self):
+        c = np.zeros(256, np.uint64)
+        check(lib().flb_flag_histogram(self.h, c.ctypes.data_as(_U64)), self.h, "flb_flag_histogram")
+        return c
+
+    def device_fields(self):
+        ptrs = [C.c_void_p() for _ in range(4)]
+        check(lib().flb_device_fields(self.h, *[C.byref(p) for p in ptrs]), self.h,
+              "flb_device_fields")
+        return [p.value for p in ptrs]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().flb_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- host set-up
+
+def build_channel_flags(nx, ny, dx, bc_left, bc_right, bc_up, bc_down, free_surface=False,
+                        obstacles=(), initial=(), x_min=0.0, y_min=0.0):
+    obs = (Rect * max(1, len(obstacles)))(*[Rect(*r) for r in obstacles])
+    ini = (Rect * max(1, len(initial)))(*[Rect(*r) for r in initial])
+    g = Channel(nx, ny, dx, x_min, y_min, int(free_surface), bc_left, bc_right, bc_up, bc_down,
+                len(obstacles), len(initial), obs, ini)
+    out = np.zeros(nx * ny, np.uint8)
+    check(lib().flb_build_channel_flags(C.byref(g), u8ptr(out)), None, "flb_build_channel_flags")
+    return out
+
+
+def seed_free_surface(flags, nx, ny):
+    flags = np.array(flags, dtype=np.uint8, copy=True)
+    check(lib().flb_seed_free_surface(u8ptr(flags), nx, ny), None, "flb_seed_free_surface")
+    return flags
+
+
+def init_free_surface_fields(flags):
+    flags = np.ascontiguousarray(flags, dtype=np.uint8)
+    n = flags.size
+    m, p, e = (np.empty(n, np.float32) for _ in range(3))
+    check(lib().flb_init_free_surface_fields(u8ptr(flags), n, fptr(m), fptr(p), fptr(e)), None,
+          "flb_init_free_surface_fields")
+    return m, p, e
+
+
+def build_culvert(nx, ny, free_surface=False, bc=(INLET, OUTLET, WALL, WALL), emb=(0, 0),
+                  opening=0, water=(0, 0), u_inlet=0.0, rho_noise=0.0, seed=0):
+    """Returns flags, rho0, u0 of a synthetic culvert (flb_build_culvert)."""
+    c = Culvert(nx, ny, int(free_surface), bc[0], bc[1], bc[2], bc[3], emb[0], emb[1], opening,
+                water[0], water[1], u_inlet, rho_noise, seed)
+    N = nx * ny
+    fl = np.zeros(N, np.uint8)
+    rho0 = np.empty(N, np.float32)
+    u0 = np.empty(2 * N, np.float32)
+    check(lib().flb_build_culvert(C.byref(c), u8ptr(fl), fptr(rho0), fptr(u0)), None,
+          "flb_build_culvert")
+    return fl, rho0, u0
+
+
+def units(**kw):
+    i = UnitsIn(**kw)
+    o = UnitsOut()
+    check(lib().flb_units(C.byref(i), C.byref(o)), None, "flb_units")
+    return o

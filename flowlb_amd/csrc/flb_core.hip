@@ -1,0 +1,553 @@
+// flb_core.hip -- handle, device memory layout and the C ABI of the FlowLB core.
+//
+// Device memory of one handle (N = nx*ny cells):
+//   f          9 planes x Np elements of the storage type (Np = N rounded up to
+//              a multiple of 64 so every plane starts 256-byte aligned)
+//   flags[2]   N bytes each; free-surface pass 4 writes the second buffer
+//   rho, u     readback fields, N and 2N floats
+//   pub_rho[2], pub_u[2]   inlet/outlet neighbour values by step parity
+//   mass, excess, phi      free surface, N floats each
+//   rec        free surface, reconstructed gas populations, 9 x N floats
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/flb_core.h"
+#include "flb_device.h"
+
+namespace flb {
+using KernelFn = void (*)(const Params);
+using MassFn = void (*)(const Params, double*);
+using ConvOutFn = void (*)(const Params, float*);
+using ConvInFn = void (*)(const Params, const float*);
+using HistFn = void (*)(const Params, unsigned long long*);
+KernelFn get_stream_collide(int storage, int odd, bool fs, bool mom);
+KernelFn get_fs1(int storage, int odd);
+KernelFn get_fs2();
+KernelFn get_fs3(int storage, int odd);
+KernelFn get_fs4();
+MassFn get_pop_mass(int storage);
+MassFn get_fs_mass();
+ConvOutFn get_pop_to_f32(int storage);
+ConvInFn get_f32_to_pop(int storage);
+HistFn get_flag_hist();
+}  // namespace flb
+
+struct flb_handle {
+  flb_options opt{};
+  uint32_t nx = 0, ny = 0;
+  uint64_t N = 0, Np = 0;
+  int elem_bytes = 4;
+  hipStream_t stream = nullptr;
+  void* f = nullptr;
+  uint8_t* flags[2] = {nullptr, nullptr};
+  int cur = 0;
+  float* rho = nullptr;
+  float* u = nullptr;
+  float* pub_rho[2] = {nullptr, nullptr};
+  float* pub_u[2] = {nullptr, nullptr};
+  float* mass = nullptr;
+  float* excess = nullptr;
+  float* phi = nullptr;
+  float* rec = nullptr;
+  float* alpha = nullptr;
+  double* partial = nullptr;
+  size_t partial_len = 0;
+  uint64_t t = 0;
+  float inv_tau = 0.0f;
+  std::vector<void*> owned;
+  std::string err;
+};
+
+namespace {
+
+constexpr int kUtilBlock = 256;
+
+int fail(flb_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+int hip_fail(flb_handle* h, hipError_t e, const char* what) {
+  return fail(h, FLB_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define FLB_HIP(h, call)                                   \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return hip_fail((h), e_, #call); \
+  } while (0)
+
+uint64_t round_up(uint64_t v, uint64_t m) { return (v + m - 1) / m * m; }
+
+int block_x(const flb_handle* h) { return h->opt.block_x > 0 ? h->opt.block_x : 256; }
+
+flb::Params params(const flb_handle* h, int bx) {
+  flb::Params p{};
+  p.nx = h->nx;
+  p.ny = h->ny;
+  p.tiles_x = uint32_t((h->nx + bx - 1) / bx);
+  p.nblocks = p.tiles_x * h->ny;
+  p.N = h->N;
+  p.Np = h->Np;
+  p.inv_tau = h->inv_tau;
+  const bool force = h->opt.use_force != 0;
+  p.fx = force ? h->opt.fx : 0.0f;
+  p.fy = force ? h->opt.fy : 0.0f;
+  p.use_force = h->opt.use_force;
+  p.collision = h->opt.collision;
+  p.smagorinsky = h->opt.smagorinsky;
+  p.gas_check_corrected = h->opt.gas_check_corrected;
+  p.mass_mode = h->opt.mass_mode;
+  p.f = h->f;
+  p.flags = h->flags[h->cur];
+  p.flags_next = h->flags[h->cur ^ 1];
+  p.rho = h->rho;
+  p.u = h->u;
+  p.pub_rho[0] = h->pub_rho[0];
+  p.pub_rho[1] = h->pub_rho[1];
+  p.pub_u[0] = h->pub_u[0];
+  p.pub_u[1] = h->pub_u[1];
+  p.mass = h->mass;
+  p.excess = h->excess;
+  p.phi = h->phi;
+  p.rec = h->rec;
+  p.alpha = h->alpha;
+  return p;
+}
+
+template <class T>
+int dev_alloc(flb_handle* h, T** ptr, size_t count) {
+  void* v = nullptr;
+  hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(h, FLB_ERR_MEMORY, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                       " bytes failed: " + hipGetErrorString(e));
+  }
+  h->owned.push_back(v);
+  *ptr = static_cast<T*>(v);
+  return FLB_OK;
+}
+
+int upload(flb_handle* h, void* dst, const void* src, size_t bytes) {
+  FLB_HIP(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+  FLB_HIP(h, hipStreamSynchronize(h->stream));
+  return FLB_OK;
+}
+
+int download(flb_handle* h, void* dst, const void* src, size_t bytes) {
+  FLB_HIP(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+  FLB_HIP(h, hipStreamSynchronize(h->stream));
+  return FLB_OK;
+}
+
+int launch(flb_handle* h, flb::KernelFn fn, const flb::Params& p, int bx) {
+  hipLaunchKernelGGL(fn, dim3(p.nblocks), dim3(bx), 0, h->stream, p);
+  FLB_HIP(h, hipGetLastError());
+  return FLB_OK;
+}
+
+// Host initial populations, initFields (src/initData.h:125-165): feq(rho0, u0)
+// evaluated in double as initEquilibriumDDF2Q9 does (:75-100), stored with
+// the odd-step pattern (storef with t = 1), DDF-shifted (f - w).
+void initial_populations(uint32_t nx, uint32_t ny, const float* rho0, const float* u0,
+                         std::vector<float>& fs) {
+  const size_t N = size_t(nx) * ny;
+  fs.assign(9 * N, 0.0f);
+  const double w[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
+                       1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+  const int cxs[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
+  const int cys[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1};
+  for (uint32_t y = 0; y < ny; ++y)
+    for (uint32_t x = 0; x < nx; ++x) {
+      const size_t n = size_t(y) * nx + x;
+      const double r = rho0 ? double(rho0[n]) : 1.0;
+      const double ux = u0 ? double(u0[n]) : 0.0, uy = u0 ? double(u0[N + n]) : 0.0;
+      const double usq = 1.5 * (ux * ux + uy * uy);
+      double feq[9];
+      for (int i = 0; i < 9; ++i) {
+        const double cu = 3.0 * (cxs[i] * ux + cys[i] * uy);
+        // w (r - 1) + w r (cu + cu^2/2 - 3u^2/2): the shifted equilibrium
+        feq[i] = w[i] * (r - 1.0) + w[i] * r * (cu + 0.5 * cu * cu - usq);
+      }
+      const uint32_t xr = x + 1 == nx ? 0 : x + 1, xl = x == 0 ? nx - 1 : x - 1;
+      const uint32_t yu = y + 1 == ny ? 0 : y + 1, yd = y == 0 ? ny - 1 : y - 1;
+      const size_t nb[9] = {n,
+                            size_t(y) * nx + xr,
+                            size_t(y) * nx + xl,
+                            size_t(yu) * nx + x,
+                            size_t(yd) * nx + x,
+                            size_t(yu) * nx + xr,
+                            size_t(yd) * nx + xl,
+                            size_t(yd) * nx + xr,
+                            size_t(yu) * nx + xl};
+      // storef at odd parity: f_i -> slot (i+1, n + c_i), f_{i+1} -> slot (i, n).
+      fs[n] = float(feq[0]);
+      for (int i = 1; i < 9; i += 2) {
+        fs[size_t(i + 1) * N + nb[i]] = float(feq[i]);
+        fs[size_t(i) * N + n] = float(feq[i + 1]);
+      }
+    }
+}
+
+int validate(const flb_options* o, uint32_t nx, uint32_t ny, std::string& why) {
+  if (!(o->tau > 0.5f) || !std::isfinite(o->tau)) return why = "tau must be > 0.5", 1;
+  if (o->storage < 0 || o->storage > 2) return why = "unknown storage format", 1;
+  if (o->collision < 0 || o->collision > 1) return why = "collision must be SRT or TRT", 1;
+  if (o->problem < 0 || o->problem > 1) return why = "problem must be monofluid or free surface", 1;
+  if (o->mass_mode < 0 || o->mass_mode > 1) return why = "unknown mass mode", 1;
+  if (o->block_x != 0 && (o->block_x < 64 || o->block_x > 1024 || o->block_x % 64))
+    return why = "block_x must be a multiple of 64 in [64, 1024]", 1;
+  if (nx < 2 || ny < 2) return why = "grid must be at least 2 x 2", 1;
+  if (uint64_t(nx) * ny > 0xffffffffull / 2) return why = "grid too large for 32-bit cell indices", 1;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void flb_default_options(flb_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->tau = 0.6f;
+  o->collision = FLB_SRT;
+  o->problem = FLB_MONOFLUID;
+  o->storage = FLB_STORAGE_FP32;
+  o->mass_mode = FLB_MASS_CONSERVATIVE;
+}
+
+const char* flb_status_string(int s) {
+  switch (s) {
+    case FLB_OK: return "ok";
+    case FLB_ERR_ARGUMENT: return "invalid argument";
+    case FLB_ERR_DEVICE: return "device error";
+    case FLB_ERR_MEMORY: return "out of memory";
+    case FLB_ERR_STATE: return "invalid state";
+    case FLB_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+  }
+}
+
+const char* flb_last_error(const flb_handle* h) { return h ? h->err.c_str() : ""; }
+
+uint64_t flb_time(const flb_handle* h) { return h ? h->t : 0; }
+
+void flb_destroy(flb_handle* h) {
+  if (!h) return;
+  if (h->stream) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipStreamDestroy(h->stream);
+  }
+  for (void* v : h->owned) (void)hipFree(v);
+  delete h;
+}
+
+int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32_t ny,
+               const float* rho0, const float* u0, flb_handle** out) {
+  if (!out) return FLB_ERR_ARGUMENT;
+  *out = nullptr;
+  if (!opt || !flags) return FLB_ERR_ARGUMENT;
+  std::string why;
+  if (validate(opt, nx, ny, why)) return FLB_ERR_ARGUMENT;
+  const uint64_t N = uint64_t(nx) * ny;
+  // The reference reads rho[n+1] at inlets and u[n-1] at outlets
+  // (lbmKernel2D.cu:336,344); those must exist.
+  if (flags[N - 1] & FLB_INLET) return FLB_ERR_ARGUMENT;
+  if ((flags[0] & FLB_OUTLET) && !(flags[0] & FLB_INLET)) return FLB_ERR_ARGUMENT;
+
+  flb_handle* h = new (std::nothrow) flb_handle();
+  if (!h) return FLB_ERR_MEMORY;
+  h->opt = *opt;
+  h->nx = nx;
+  h->ny = ny;
+  h->N = N;
+  h->Np = round_up(N, 64);
+  h->elem_bytes = opt->storage == FLB_STORAGE_FP32 ? 4 : 2;
+  h->inv_tau = 1.0f / opt->tau;
+  auto bail = [&](int rc) {
+    flb_destroy(h);
+    return rc;
+  };
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return bail(FLB_ERR_DEVICE);
+  }
+  if (opt->device < 0 || opt->device >= ndev) return bail(FLB_ERR_ARGUMENT);
+  if (hipSetDevice(opt->device) != hipSuccess) return bail(FLB_ERR_DEVICE);
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(FLB_ERR_DEVICE);
+
+  int rc = FLB_OK;
+  uint8_t* fbytes = nullptr;
+  if ((rc = dev_alloc(h, &fbytes, 9 * h->Np * h->elem_bytes))) return bail(rc);
+  h->f = fbytes;
+  const bool fs = opt->problem == FLB_FREE_SURFACE;
+  if ((rc = dev_alloc(h, &h->flags[0], N))) return bail(rc);
+  if ((rc = dev_alloc(h, &h->flags[1], fs ? N : 1))) return bail(rc);
+  if ((rc = dev_alloc(h, &h->rho, N))) return bail(rc);
+  if ((rc = dev_alloc(h, &h->u, 2 * N))) return bail(rc);
+  for (int k = 0; k < 2; ++k) {
+    if ((rc = dev_alloc(h, &h->pub_rho[k], N))) return bail(rc);
+    if ((rc = dev_alloc(h, &h->pub_u[k], 2 * N))) return bail(rc);
+  }
+  if (fs) {
+    if ((rc = dev_alloc(h, &h->mass, N))) return bail(rc);
+    if ((rc = dev_alloc(h, &h->excess, N))) return bail(rc);
+    if ((rc = dev_alloc(h, &h->phi, N))) return bail(rc);
+    if ((rc = dev_alloc(h, &h->rec, 9 * N))) return bail(rc);
+  }
+  const flb::Params pu = params(h, kUtilBlock);
+  h->partial_len = pu.nblocks;
+  if ((rc = dev_alloc(h, &h->partial, h->partial_len))) return bail(rc);
+
+  // Fields: rho = rho0 (or 1), u = u0 (or 0); the publication buffers start
+  // with the same values (what the reference's fields hold before step 0).
+  std::vector<float> hr(N, 1.0f), hu(2 * N, 0.0f);
+  if (rho0) std::memcpy(hr.data(), rho0, N * sizeof(float));
+  if (u0) std::memcpy(hu.data(), u0, 2 * N * sizeof(float));
+  if ((rc = upload(h, h->flags[0], flags, N))) return bail(rc);
+  if (fs && (rc = upload(h, h->flags[1], flags, N))) return bail(rc);
+  if ((rc = upload(h, h->rho, hr.data(), N * sizeof(float)))) return bail(rc);
+  if ((rc = upload(h, h->u, hu.data(), 2 * N * sizeof(float)))) return bail(rc);
+  for (int k = 0; k < 2; ++k) {
+    if ((rc = upload(h, h->pub_rho[k], hr.data(), N * sizeof(float)))) return bail(rc);
+    if ((rc = upload(h, h->pub_u[k], hu.data(), 2 * N * sizeof(float)))) return bail(rc);
+  }
+  if (fs) {
+    FLB_HIP(h, hipMemsetAsync(h->mass, 0, N * sizeof(float), h->stream));
+    FLB_HIP(h, hipMemsetAsync(h->excess, 0, N * sizeof(float), h->stream));
+    FLB_HIP(h, hipMemsetAsync(h->phi, 0, N * sizeof(float), h->stream));
+    FLB_HIP(h, hipMemsetAsync(h->rec, 0, 9 * N * sizeof(float), h->stream));
+  }
+  // Populations: shifted feq, converted to the storage format on the device.
+  {
+    std::vector<float> fsh;
+    initial_populations(nx, ny, rho0, u0, fsh);
+    float* staging = nullptr;
+    if (hipMalloc(&staging, fsh.size() * sizeof(float)) != hipSuccess) {
+      (void)hipGetLastError();
+      return bail(FLB_ERR_MEMORY);
+    }
+    int r2 = upload(h, staging, fsh.data(), fsh.size() * sizeof(float));
+    if (!r2) {
+      const flb::Params p = params(h, kUtilBlock);
+      hipLaunchKernelGGL(flb::get_f32_to_pop(opt->storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
+                         h->stream, p, staging);
+      hipError_t e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) r2 = FLB_ERR_DEVICE;
+    }
+    (void)hipFree(staging);
+    if (r2) return bail(r2);
+  }
+  *out = h;
+  return FLB_OK;
+}
+
+int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const float* excess) {
+  if (!h || !mass || !phi) return h ? fail(h, FLB_ERR_ARGUMENT, "mass and phi required") : FLB_ERR_ARGUMENT;
+  if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "not a free-surface handle");
+  const size_t b = h->N * sizeof(float);
+  int rc;
+  if ((rc = upload(h, h->mass, mass, b))) return rc;
+  if ((rc = upload(h, h->phi, phi, b))) return rc;
+  if (excess) return upload(h, h->excess, excess, b);
+  FLB_HIP(h, hipMemsetAsync(h->excess, 0, b, h->stream));
+  return FLB_OK;
+}
+
+// One lattice step on the handle's stream.  ev (optional) receives per-kernel
+// event pairs: ev[2k], ev[2k+1] around kernel k.
+static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
+  const int bx = block_x(h);
+  const int odd = int(h->t & 1u);
+  const int st = h->opt.storage;
+  const bool fs = h->opt.problem == FLB_FREE_SURFACE;
+  flb::Params p = params(h, bx);
+  auto run = [&](int k, flb::KernelFn fn) -> int {
+    if (ev) FLB_HIP(h, hipEventRecord(ev[2 * k], h->stream));
+    int rc = launch(h, fn, p, bx);
+    if (rc) return rc;
+    if (ev) FLB_HIP(h, hipEventRecord(ev[2 * k + 1], h->stream));
+    return FLB_OK;
+  };
+  int rc;
+  if (fs) {
+    if ((rc = run(FLB_K_FS1, flb::get_fs1(st, odd)))) return rc;
+    if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, true)))) return rc;
+    if ((rc = run(FLB_K_FS2, flb::get_fs2()))) return rc;
+    if ((rc = run(FLB_K_FS3, flb::get_fs3(st, odd)))) return rc;
+    if ((rc = run(FLB_K_FS4, flb::get_fs4()))) return rc;
+    h->cur ^= 1;
+  } else {
+    if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, false, moments)))) return rc;
+  }
+  h->t += 1;
+  return FLB_OK;
+}
+
+int flb_step(flb_handle* h, uint64_t nsteps) {
+  if (!h) return FLB_ERR_ARGUMENT;
+  for (uint64_t k = 0; k < nsteps; ++k) {
+    const int rc = enqueue_step(h, k + 1 == nsteps, nullptr);
+    if (rc) return rc;
+  }
+  return FLB_OK;
+}
+
+int flb_sync(flb_handle* h) {
+  if (!h) return FLB_ERR_ARGUMENT;
+  FLB_HIP(h, hipStreamSynchronize(h->stream));
+  return FLB_OK;
+}
+
+int flb_step_timed(flb_handle* h, uint64_t nsteps, float* total_ms, float* per_kernel_ms) {
+  if (!h) return FLB_ERR_ARGUMENT;
+  if (!per_kernel_ms) {
+    hipEvent_t a, b;
+    FLB_HIP(h, hipEventCreate(&a));
+    FLB_HIP(h, hipEventCreate(&b));
+    FLB_HIP(h, hipEventRecord(a, h->stream));
+    int rc = flb_step(h, nsteps);
+    if (!rc) {
+      FLB_HIP(h, hipEventRecord(b, h->stream));
+      FLB_HIP(h, hipEventSynchronize(b));
+      float ms = 0.0f;
+      FLB_HIP(h, hipEventElapsedTime(&ms, a, b));
+      if (total_ms) *total_ms = ms;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+  }
+  hipEvent_t ev[2 * FLB_NUM_KERNELS];
+  for (auto& e : ev) FLB_HIP(h, hipEventCreate(&e));
+  for (int k = 0; k < FLB_NUM_KERNELS; ++k) per_kernel_ms[k] = 0.0f;
+  float tot = 0.0f;
+  int rc = FLB_OK;
+  const bool fs = h->opt.problem == FLB_FREE_SURFACE;
+  for (uint64_t s = 0; s < nsteps && !rc; ++s) {
+    rc = enqueue_step(h, s + 1 == nsteps, ev);
+    if (rc) break;
+    FLB_HIP(h, hipEventSynchronize(ev[2 * (fs ? FLB_K_FS4 : FLB_K_STREAM_COLLIDE) + 1]));
+    for (int k = 0; k < FLB_NUM_KERNELS; ++k) {
+      if (!fs && k != FLB_K_STREAM_COLLIDE) continue;
+      float ms = 0.0f;
+      FLB_HIP(h, hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+      per_kernel_ms[k] += ms;
+      tot += ms;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (total_ms) *total_ms = tot;
+  return rc;
+}
+
+int flb_read(flb_handle* h, float* u, float* rho, uint8_t* flags, float* phi) {
+  if (!h) return FLB_ERR_ARGUMENT;
+  const size_t N = h->N;
+  int rc;
+  if (u && (rc = download(h, u, h->u, 2 * N * sizeof(float)))) return rc;
+  if (rho && (rc = download(h, rho, h->rho, N * sizeof(float)))) return rc;
+  if (flags && (rc = download(h, flags, h->flags[h->cur], N))) return rc;
+  if (phi) {
+    if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "phi needs free surface");
+    if ((rc = download(h, phi, h->phi, N * sizeof(float)))) return rc;
+  }
+  return FLB_OK;
+}
+
+int flb_read_fs(flb_handle* h, float* mass, float* excess) {
+  if (!h) return FLB_ERR_ARGUMENT;
+  if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "not a free-surface handle");
+  int rc;
+  if (mass && (rc = download(h, mass, h->mass, h->N * sizeof(float)))) return rc;
+  if (excess && (rc = download(h, excess, h->excess, h->N * sizeof(float)))) return rc;
+  return FLB_OK;
+}
+
+int flb_read_populations(flb_handle* h, float* f) {
+  if (!h || !f) return FLB_ERR_ARGUMENT;
+  float* tmp = nullptr;
+  FLB_HIP(h, hipMalloc(&tmp, 9 * h->N * sizeof(float)));
+  const flb::Params p = params(h, kUtilBlock);
+  hipLaunchKernelGGL(flb::get_pop_to_f32(h->opt.storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
+                     h->stream, p, tmp);
+  hipError_t e = hipGetLastError();
+  int rc = e == hipSuccess ? download(h, f, tmp, 9 * h->N * sizeof(float)) : hip_fail(h, e, "pop_to_f32");
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int flb_write_populations(flb_handle* h, const float* f) {
+  if (!h || !f) return FLB_ERR_ARGUMENT;
+  const size_t N = h->N;
+  std::vector<float> fsh(9 * N);
+  for (int q = 0; q < 9; ++q) {
+    const float w = q == 0 ? 4.0f / 9.0f : (q < 5 ? 1.0f / 9.0f : 1.0f / 36.0f);
+    for (size_t n = 0; n < N; ++n) fsh[q * N + n] = f[q * N + n] - w;
+  }
+  float* tmp = nullptr;
+  FLB_HIP(h, hipMalloc(&tmp, 9 * N * sizeof(float)));
+  int rc = upload(h, tmp, fsh.data(), 9 * N * sizeof(float));
+  if (!rc) {
+    const flb::Params p = params(h, kUtilBlock);
+    hipLaunchKernelGGL(flb::get_f32_to_pop(h->opt.storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
+                       h->stream, p, tmp);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = hip_fail(h, e, "f32_to_pop");
+  }
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int flb_device_fields(flb_handle* h, void** d_u, void** d_rho, void** d_flags, void** d_phi) {
+  if (!h) return FLB_ERR_ARGUMENT;
+  if (d_u) *d_u = h->u;
+  if (d_rho) *d_rho = h->rho;
+  if (d_flags) *d_flags = h->flags[h->cur];
+  if (d_phi) *d_phi = h->phi;
+  return FLB_OK;
+}
+
+int flb_mass(flb_handle* h, double* total) {
+  if (!h || !total) return FLB_ERR_ARGUMENT;
+  const flb::Params p = params(h, kUtilBlock);
+  const bool fs = h->opt.problem == FLB_FREE_SURFACE;
+  hipLaunchKernelGGL(fs ? flb::get_fs_mass() : flb::get_pop_mass(h->opt.storage), dim3(p.nblocks),
+                     dim3(kUtilBlock), 0, h->stream, p, h->partial);
+  FLB_HIP(h, hipGetLastError());
+  std::vector<double> part(h->partial_len);
+  int rc = download(h, part.data(), h->partial, part.size() * sizeof(double));
+  if (rc) return rc;
+  double s = 0.0;
+  for (double v : part) s += v;
+  *total = fs ? s : s + double(h->N);  // monofluid: shifted sum + N * sum(w)
+  return FLB_OK;
+}
+
+int flb_flag_histogram(flb_handle* h, uint64_t* counts) {
+  if (!h || !counts) return FLB_ERR_ARGUMENT;
+  unsigned long long* d = nullptr;
+  FLB_HIP(h, hipMalloc(&d, 256 * sizeof(unsigned long long)));
+  int rc = FLB_OK;
+  hipError_t e = hipMemsetAsync(d, 0, 256 * sizeof(unsigned long long), h->stream);
+  if (e == hipSuccess) {
+    const flb::Params p = params(h, kUtilBlock);
+    hipLaunchKernelGGL(flb::get_flag_hist(), dim3(p.nblocks), dim3(kUtilBlock), 0, h->stream, p, d);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) rc = hip_fail(h, e, "flag histogram");
+  else rc = download(h, counts, d, 256 * sizeof(unsigned long long));
+  (void)hipFree(d);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// flb_device.h -- device-side building blocks shared by the FlowLB HIP kernels:
+// D2Q9 lattice constants, the population storage codecs and the kernel
+// parameter block.  gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flb {
+
+// D2Q9 velocities in the reference order (src/cuda/cudaInitData.cu:76-78);
+// opposite pairs (1,2) (3,4) (5,6) (7,8).
+__host__ __device__ constexpr int cx(int i) {
+  return i == 1 || i == 5 || i == 7 ? 1 : (i == 2 || i == 6 || i == 8 ? -1 : 0);
+}
+__host__ __device__ constexpr int cy(int i) {
+  return i == 3 || i == 5 || i == 8 ? 1 : (i == 4 || i == 6 || i == 7 ? -1 : 0);
+}
+// Weights in float as the host builds them (src/ui/app.cpp:288).
+__host__ __device__ constexpr float wgt(int i) {
+  return i == 0 ? 4.0f / 9.0f : (i < 5 ? 1.0f / 9.0f : 1.0f / 36.0f);
+}
+
+enum : uint8_t {
+  kFluid = 0x01, kGas = 0x02, kInterface = 0x04, kWall = 0x08, kMovingWall = 0x10,
+  kInlet = 0x20, kOutlet = 0x40, kGasInterface = 0x80,
+  kFGI = 0x07, kIF = 0x05, kIG = 0x06, kInletFluid = 0x21
+};
+
+// ---------------------------------------------------------------- storage
+// All formats hold the DDF-shifted population f_i - w_i; arithmetic is FP32.
+struct StoreF32 {
+  using T = float;
+  static constexpr int kBytes = 4;
+  __device__ static float load(const T* p, size_t i) { return p[i]; }
+  __device__ static void store(T* p, size_t i, float v) { p[i] = v; }
+};
+
+// IEEE binary16, scaled by 2^15 so that the shifted populations (|f - w| < 1)
+// sit in the normal range.
+struct StoreF16S {
+  using T = _Float16;
+  static constexpr int kBytes = 2;
+  __device__ static float load(const T* p, size_t i) { return float(p[i]) * 3.0517578125e-05f; }
+  __device__ static void store(T* p, size_t i, float v) { p[i] = _Float16(v * 32768.0f); }
+};
+
+// bfloat16 (upper half of an FP32), round to nearest even on store.
+struct StoreBF16S {
+  using T = uint16_t;
+  static constexpr int kBytes = 2;
+  __device__ static float load(const T* p, size_t i) {
+    return __uint_as_float(uint32_t(p[i]) << 16);
+  }
+  __device__ static void store(T* p, size_t i, float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    p[i] = uint16_t(u >> 16);
+  }
+};
+
+// ---------------------------------------------------------------- params
+struct Params {
+  uint32_t nx, ny;
+  uint32_t tiles_x;      // blocks per row
+  uint32_t nblocks;      // total blocks of a full-grid launch
+  uint64_t N;            // cells
+  uint64_t Np;           // plane stride of the population array (elements)
+  float inv_tau, fx, fy;
+  int32_t use_force, collision, smagorinsky, gas_check_corrected, mass_mode;
+  void* f;               // populations, 9 planes of Np elements of the storage type
+  uint8_t* flags;        // current flags
+  uint8_t* flags_next;   // flags written by the last free-surface pass
+  float* rho;            // readback density (N)
+  float* u;              // readback velocity (2N)
+  float* pub_rho[2];     // densities published for inlet cells, by parity
+  float* pub_u[2];       // velocities published for outlet cells, by parity
+  float* mass;
+  float* excess;
+  float* phi;
+  float* rec;            // reconstructed gas populations, 9 planes of N
+  const float* alpha;    // Brinkman solid fraction or nullptr
+};
+
+// Block -> tile mapping.  Workgroups are dispatched round-robin over the 8
+// XCDs; remapping gives each XCD a contiguous range of tiles (rows), so the
+// row-shifted loads and stores of neighbouring tiles meet in one L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb >> 3, r = nb & 7u, xcd = b & 7u, k = b >> 3;
+  return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
+}  // namespace flb

@@ -1,0 +1,234 @@
+/*
+ * flb_core.h -- C ABI of the MI355X-native FlowLB lattice-Boltzmann core.
+ *
+ * The FlowLB C++ host runs its solver through one templated call,
+ * FLB::h_launchCudaCalculations2D<T> (src/cuda/runCuda.cuh:26), which does the
+ * set-up, the per-step loop and the readback internally.  This library splits
+ * that call into a handle with explicit set-up, advance and readback entry
+ * points, with plain pointers and sizes only, integer status codes and no
+ * process exit on error.  Every entry point below names the reference code it
+ * replaces.
+ *
+ * Conventions (same as the reference):
+ *   - cell index n = x + y*nx, y = 0 at the bed;
+ *   - vector fields are stored [x components | y components] (2*N floats);
+ *   - flags are one byte per cell with the bit values of FLB::TypesNodes
+ *     (src/utils.h:91-108), see FLB_* below;
+ *   - the D2Q9 velocity order is 0:(0,0) 1:(+1,0) 2:(-1,0) 3:(0,+1) 4:(0,-1)
+ *     5:(+1,+1) 6:(-1,-1) 7:(+1,-1) 8:(-1,+1) (src/cuda/cudaInitData.cu:76-78).
+ *
+ * Threading: one handle per host thread; calls on one handle are serialised by
+ * the caller.  flb_step only enqueues work on the handle's HIP stream; the
+ * readback calls (flb_read*, flb_mass) and flb_sync wait for it.
+ */
+#ifndef FLB_CORE_H
+#define FLB_CORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLB_ABI_VERSION 1
+
+/* Status codes (the reference calls exit(EXIT_FAILURE), src/cuda/cudaUtils.cuh:17-26). */
+enum {
+  FLB_OK = 0,
+  FLB_ERR_ARGUMENT = 1,    /* invalid argument or option */
+  FLB_ERR_DEVICE = 2,      /* HIP runtime error (no device, launch failure, ...) */
+  FLB_ERR_MEMORY = 3,      /* device or host allocation failed */
+  FLB_ERR_STATE = 4,       /* call not valid in the handle's state */
+  FLB_ERR_UNSUPPORTED = 5  /* option combination not implemented */
+};
+
+/* Cell flags, FLB::TypesNodes (src/utils.h:91-108). */
+enum {
+  FLB_FLUID = 0x01, FLB_GAS = 0x02, FLB_INTERFACE = 0x04, FLB_WALL = 0x08,
+  FLB_MOVING_WALL = 0x10, FLB_INLET = 0x20, FLB_OUTLET = 0x40, FLB_GAS_INTERFACE = 0x80
+};
+
+/* Population storage in device memory.  All arithmetic is FP32; the 16-bit
+   formats store f_i - w_i (DDF-shifted), FP16 scaled by 2^15. */
+enum { FLB_STORAGE_FP32 = 0, FLB_STORAGE_FP16S = 1, FLB_STORAGE_BF16S = 2 };
+/* OptionsCalculation::typeProblem (src/io/reader.h:104). */
+enum { FLB_MONOFLUID = 0, FLB_FREE_SURFACE = 1 };
+/* OptionsCalculation::collisionOperator: 0 = SRT, 1 = TRT (src/cuda/lbmKernel2D.cu:444-511). */
+enum { FLB_SRT = 0, FLB_TRT = 1 };
+/* Excess-mass accounting of the free-surface passes.
+   FLB_MASS_AS_WRITTEN: the reference's rules (receivers counted on the
+   pre-pass flags, fresh own fill level in the exchange weight, fluid cells
+   exchange over every link).  FLB_MASS_CONSERVATIVE: mass crosses only
+   fluid/interface links with symmetric weights and excess goes only to cells
+   that pull it, so total mass is conserved to round-off. */
+enum { FLB_MASS_AS_WRITTEN = 0, FLB_MASS_CONSERVATIVE = 1 };
+
+/* Solver options: the lattice-unit content of OptionsCalculation
+   (src/io/reader.h:98-127) after FLB::h_initConstantDataCuda2D
+   (src/cuda/cudaInitData.cu:57-108) converted it. */
+typedef struct flb_options {
+  float tau;                   /* relaxation time; 1/tau is formed in float */
+  float fx, fy;                /* body force per cell (Guo forcing), lattice units */
+  int32_t use_force;           /* USE_VOLUMETRIC_FORCE */
+  int32_t collision;           /* FLB_SRT / FLB_TRT */
+  int32_t smagorinsky;         /* SUBGRID_TURBULENCE_MODEL */
+  int32_t problem;             /* FLB_MONOFLUID / FLB_FREE_SURFACE */
+  int32_t storage;             /* FLB_STORAGE_* (PRECISION 32 -> FP32, 16 -> FP16S) */
+  int32_t gas_check_corrected; /* 0: gas-reconstruction neighbour test as written
+                                  (src/cuda/cellCalculations.cuh:250-259); 1: test
+                                  the neighbour the population comes from */
+  int32_t mass_mode;           /* FLB_MASS_* */
+  int32_t block_x;             /* threads per block along x (64..1024, multiple of 64); 0 = 256 */
+  int32_t device;              /* HIP device ordinal */
+} flb_options;
+
+typedef struct flb_handle flb_handle;
+
+/* Fills *opt with the defaults (SRT, FP32 storage, monofluid, tau = 0.6). */
+void flb_default_options(flb_options* opt);
+
+/* Set-up.  Replaces the set-up half of h_launchCudaCalculations2D
+   (src/cuda/runCuda.cu:31-85): initFields (src/initData.h:125-165),
+   h_initConstantDataCuda2D (src/cuda/cudaInitData.cu:57-108) and
+   h_initDataCuda2D (src/cuda/cudaInitData.cuh:205-240).  flags: N bytes.
+   rho0 (N floats) may be NULL for rho = 1; u0 (2N floats, [ux | uy]) may be
+   NULL for u = 0 (inlet cells then need u0 for their prescribed velocity).
+   Populations start at feq(rho0, u0) stored with the odd-step pattern. */
+int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32_t ny,
+               const float* rho0, const float* u0, flb_handle** out);
+
+/* Free-surface fields after initFreeSurfaceFields (src/initData.h:175-206).
+   excess may be NULL (zero). */
+int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const float* excess);
+
+/* Advance nsteps lattice steps.  Replaces the step loops
+   h_runCudaMonoFluidNoGraphics2D / h_runCudaFreeSurfaceNoGraphics2D
+   (src/cuda/runCuda.cu:473-490, 506-527): one d_StreamCollide2D per step, or
+   d_FreeSurface2D_1 -> d_StreamCollide2D -> d_FreeSurface2D_2 -> _3 -> _4.
+   The handle owns the time counter t (parity of the esoteric pull).  The
+   density/velocity readback fields hold the values of the last step. */
+int flb_step(flb_handle* h, uint64_t nsteps);
+
+/* Waits for all work enqueued on the handle. */
+int flb_sync(flb_handle* h);
+
+/* Readback into caller-owned host buffers; every pointer may be NULL.
+   Replaces CudaUtils::copyDataFromDevice (src/cuda/cudaUtils.cuh:42-53) with
+   the fields the run loops save (src/cuda/runCuda.cu:483-487, 518-523):
+   u (2N), rho (N), flags (N), phi (N, free surface only). */
+int flb_read(flb_handle* h, float* u, float* rho, uint8_t* flags, float* phi);
+
+/* Free-surface mass and excess-mass fields (N floats each, NULL allowed). */
+int flb_read_fs(flb_handle* h, float* mass, float* excess);
+
+/* Raw population array f[q*N + n] in the esoteric-pull slot layout, as
+   unshifted FP32 values (9N floats). */
+int flb_read_populations(flb_handle* h, float* f);
+int flb_write_populations(flb_handle* h, const float* f);
+
+/* Device pointers of the fields, for a HIP-GL interop consumer
+   (the reference maps u onto a GL buffer, src/cuda/runCuda.cu:137-147).
+   Any output pointer may be NULL.  They stay valid until flb_destroy. */
+int flb_device_fields(flb_handle* h, void** d_u, void** d_rho, void** d_flags, void** d_phi);
+
+/* Total fluid mass in FP64: stored mass plus the excess-mass shares that the
+   next step pulls in (free surface), or the sum of all populations
+   (monofluid).  No reference counterpart (the reference has no mass
+   accounting). */
+int flb_mass(flb_handle* h, double* total);
+
+/* Lattice time t (number of steps done). */
+uint64_t flb_time(const flb_handle* h);
+
+/* Enqueue nsteps steps between two HIP events on the handle's stream and
+   return the elapsed device time in ms.  With per_kernel_ms non-NULL (array of
+   FLB_NUM_KERNELS floats) each kernel launch is bracketed by its own events and
+   the summed device time per kernel is returned instead. */
+enum { FLB_K_STREAM_COLLIDE = 0, FLB_K_FS1 = 1, FLB_K_FS2 = 2, FLB_K_FS3 = 3, FLB_K_FS4 = 4,
+       FLB_NUM_KERNELS = 5 };
+int flb_step_timed(flb_handle* h, uint64_t nsteps, float* total_ms, float* per_kernel_ms);
+
+/* Histogram of flags (256 counts), computed on the device. */
+int flb_flag_histogram(flb_handle* h, uint64_t* counts);
+
+/* Last error message of the handle ("" if none), and a name for a status. */
+const char* flb_last_error(const flb_handle* h);
+const char* flb_status_string(int status);
+
+/* Frees every device buffer of the handle (src/cuda/runCuda.cu:111-127). */
+void flb_destroy(flb_handle* h);
+
+/* ------------------------------------------------------------ host set-up */
+
+/* Rectangle in SI units: centre (x0, y0), height (y), width (x)
+   (RectangleShape, src/geometry/shapes.cpp:58-126). */
+typedef struct flb_rect { double x0, y0, height, width; } flb_rect;
+
+/* Flat-bed channel whose cross-drainage work spans the whole domain. */
+typedef struct flb_channel {
+  uint32_t nx, ny;
+  double dx;                 /* SIZE */
+  double x_min, y_min;
+  int32_t free_surface;      /* default node GAS instead of FLUID */
+  uint8_t bc_left, bc_right, bc_up, bc_down;  /* FLB_INLET/OUTLET/WALL/FLUID */
+  uint32_t n_obstacles, n_initial;
+  const flb_rect* obstacles; /* OBSTACLE(k) -> WALL */
+  const flb_rect* initial;   /* INITIAL_CONDITION_FLAG(k) -> FLUID */
+} flb_channel;
+
+/* Flags of a channel, initData (src/initData.h:220-333): default node,
+   boundary rows/columns (down, up, left, right priority), obstacles, initial
+   fluid; for free surface followed by flb_seed_free_surface. */
+int flb_build_channel_flags(const flb_channel* g, uint8_t* flags);
+
+/* initFreeSurfaceFlags (src/initData.cpp:13-37) in index order. */
+int flb_seed_free_surface(uint8_t* flags, uint32_t nx, uint32_t ny);
+
+/* initFreeSurfaceFields (src/initData.h:175-206). excess may be NULL. */
+int flb_init_free_surface_fields(const uint8_t* flags, size_t n, float* mass, float* phi,
+                                 float* excess);
+
+/* Synthetic culvert in lattice units: walls/inlet/outlet on the sides, an
+   embankment of WALL cells over columns [emb_x0, emb_x1] above a culvert
+   opening of `opening` cells at the bed, optional initial water block over
+   columns [1, water_x1] and rows [1, water_depth].  rho0 = 1 + U(-noise,
+   noise) drawn with std::mt19937_64(seed) in index order (rho0 may be NULL);
+   u0 = (u_inlet, 0) at inlet cells, 0 elsewhere (u0 may be NULL). */
+typedef struct flb_culvert {
+  uint32_t nx, ny;
+  int32_t free_surface;
+  uint8_t bc_left, bc_right, bc_up, bc_down;
+  uint32_t emb_x0, emb_x1, opening;
+  uint32_t water_x1, water_depth;  /* water_x1 = 0: no water block */
+  float u_inlet;
+  double rho_noise;
+  uint64_t seed;
+} flb_culvert;
+int flb_build_culvert(const flb_culvert* c, uint8_t* flags, float* rho0, float* u0);
+
+/* Unit conversion, FLB::Units (src/utils.cpp:7-81) as used by
+   h_initConstantDataCuda2D (src/cuda/cudaInitData.cu:57-84): lattice tau and
+   body force from SI inputs. */
+typedef struct flb_units_in {
+  double length_si;          /* x extent of the domain (m) */
+  uint32_t nx;               /* nodes along x */
+  double ref_velocity_si, ref_velocity_lb, density_si;
+  double relaxation_time;    /* used when != 0 */
+  double kinematic_viscosity;
+  int32_t use_force;
+  double acceleration_x, acceleration_y;       /* precedence when != 0 */
+  double volumetric_force_x, volumetric_force_y;
+  double velocity_si_x, velocity_si_y;
+} flb_units_in;
+typedef struct flb_units_out {
+  float tau, inv_tau, fx, fy;
+  double velocity_lb_x, velocity_lb_y;
+  double length_scale, time_scale, mass_scale;
+} flb_units_out;
+int flb_units(const flb_units_in* in, flb_units_out* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLB_CORE_H */

@@ -166,7 +166,8 @@ struct Sim {
   std::vector<uint8_t> flags;
   // scratch
   std::vector<T> rho_snap, u_snap, rec;
-  std::vector<uint8_t> rec_mask, flags_snap;
+  std::vector<uint16_t> rec_mask;
+  std::vector<uint8_t> flags_snap;
 
   int threads() const { return p.threads > 0 ? p.threads : 1; }
 
@@ -367,7 +368,7 @@ struct Sim {
   // every outgoing-population read sees the pre-kernel array.
   void fs1() {
     const uint64_t tt = t;
-    std::fill(rec_mask.begin(), rec_mask.end(), uint8_t(0));
+    std::fill(rec_mask.begin(), rec_mask.end(), uint16_t(0));
 #pragma omp parallel for schedule(static) num_threads(threads())
     for (int64_t yy = 0; yy < int64_t(ny); yy++) {
       const uint32_t y = uint32_t(yy);
@@ -438,12 +439,12 @@ struct Sim {
           // storeReconstructedGasf (cellCalculations.cuh:250-259): rc[k] replaces
           // the incoming population k.  As written the gate is the neighbour in
           // direction +c_k; corrected it is the source neighbour at -c_k.
-          uint8_t mask = 0;
+          uint16_t mask = 0;
           for (int k = 1; k < 9; k++) {
             const int opp = (k & 1) ? k + 1 : k - 1;
             const uint8_t gate = p.gas_check_corrected ? nfl[opp] : nfl[k];
             if (gate == GAS) {
-              mask |= uint8_t(1u << k);
+              mask |= uint16_t(1u << k);
               rec[k * N + n] = rc[k];
             }
           }
