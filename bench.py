@@ -93,6 +93,150 @@ def measure(case, storage, steps, warmup, profile_steps, **opt):
                 finite=bool(mass_ok))
 
 
+def brinkman_window(case, seed=None):
+    x0, x1, y0, y1 = case.extra["window"]
+    rng = np.random.default_rng(case.seed if seed is None else seed)
+    alpha = np.zeros((case.ny, case.nx), np.float32)
+    alpha[y0:y1 + 1, x0:x1 + 1] = rng.uniform(0, 1, (y1 - y0 + 1, x1 - x0 + 1))
+    return alpha.ravel()
+
+
+def bench_adjoint(args):
+    """Gradient of the time-averaged outlet discharge w.r.t. the 64x64 Brinkman
+    window and the inlet velocity on the 1024x256 culvert: GPU time per
+    gradient, cost relative to the forward run, and accuracy against the FP64
+    oracle adjoint and central finite differences of the FP64 objective."""
+    from flowlb_amd import cases, core
+    case = cases.ADJOINT
+    steps = case.steps
+    t0, t1 = case.extra["t_avg"]
+    col = case.nx - 2
+    ckpt = case.extra["ckpt"]
+    flags, rho0, u0 = case.build()
+    alpha = brinkman_window(case)
+    with case.solver() as s:
+        s.set_brinkman(alpha)
+        s.objective(200, col, 0, 199)            # warm-up
+        plain_ms = s.step_timed(steps)
+        s.close()
+    with case.solver() as s:
+        s.set_brinkman(alpha)
+        t = time.perf_counter()
+        J_fwd = s.objective(steps, col, t0, t1)
+        obj_s = time.perf_counter() - t
+        s.adjoint(200, col, 100, 199, ckpt)      # warm-up
+        t = time.perf_counter()
+        J, g, duin, phases = s.adjoint(steps, col, t0, t1, ckpt)
+        adj_s = time.perf_counter() - t
+    out = {"metric": METRIC, "value": round(adj_s, 4), "unit": "s/gradient", "n_gpus": 1,
+           "steps": steps, "warmup": 1, "ms_per_step": round(adj_s * 1e3 / steps, 5),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic",
+           "config": {"workload": case.name, "nx": case.nx, "ny": case.ny, "window": "64x64",
+                      "checkpoint_every": ckpt, "average_steps": [t0, t1]},
+           "J": J, "forward_ms": round(plain_ms, 3),
+           "adjoint_over_forward": round(adj_s * 1e3 / plain_ms, 3),
+           "phases_ms": [round(v, 3) for v in phases]}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle
+        a64 = alpha.astype(np.float64)
+        mk = lambda a, uin=None: pyoracle.Oracle(  # noqa: E731
+            flags, case.nx, case.ny, case.tau, use_force=True, fx=case.force[0], fy=case.force[1],
+            rho0=rho0.astype(np.float64), u0=(u0 if uin is None else _uin(u0, flags, uin)).astype(np.float64),
+            alpha=a, threads=min(16, os.cpu_count() or 1))
+        t = time.perf_counter()
+        Jr, gr, duinr = mk(a64).adjoint(steps, col, t0, t1, ckpt)
+        cpu_s = time.perf_counter() - t
+        x0, x1, y0, y1 = case.extra["window"]
+        win = np.zeros((case.ny, case.nx), bool)
+        win[y0:y1 + 1, x0:x1 + 1] = True
+        win = win.ravel()
+        out["vs_fp64_adjoint"] = {
+            "J_rel": abs(J - Jr) / abs(Jr),
+            "dJ_dalpha_relL2_window": float(np.linalg.norm(g[win] - gr[win]) / np.linalg.norm(gr[win])),
+            "dJ_duin_rel": abs(duin - duinr) / abs(duinr)}
+        rng = np.random.default_rng(1234)
+        idx = rng.choice(np.flatnonzero(win), args.fd_cells, replace=False)
+        h = 1e-3
+        errs, errs64 = [], []
+        for n in idx:
+            ap, am = a64.copy(), a64.copy()
+            ap[n] += h
+            am[n] -= h
+            fd = (mk(ap).objective(steps, col, t0, t1) - mk(am).objective(steps, col, t0, t1)) / (2 * h)
+            errs.append(abs(g[n] - fd) / abs(fd))
+            errs64.append(abs(gr[n] - fd) / abs(fd))
+        uin = float(np.float32(case.u_inlet))
+        hu = 1e-5
+        fdu = (mk(a64, uin + hu).objective(steps, col, t0, t1)
+               - mk(a64, uin - hu).objective(steps, col, t0, t1)) / (2 * hu)
+        out["vs_finite_differences"] = {
+            "cells": int(args.fd_cells), "h": h, "max_rel_err_gpu": float(max(errs)),
+            "median_rel_err_gpu": float(np.median(errs)), "max_rel_err_fp64_adjoint": float(max(errs64)),
+            "duin_rel_err_gpu": abs(duin - fdu) / abs(fdu), "duin_rel_err_fp64": abs(duinr - fdu) / abs(fdu)}
+        out["cpu_baseline"] = {"value": round(cpu_s, 3), "unit": "s/gradient",
+                               "cores": min(16, os.cpu_count() or 1), "kind": "port",
+                               "sample": "FP64 oracle adjoint of the full run with OpenMP threads"}
+    print(json.dumps(out), flush=True)
+
+
+def _uin(u0, flags, v):
+    u = u0.copy()
+    u[:flags.size][(flags & 0x20) != 0] = v
+    return u
+
+
+def bench_batch(args):
+    """256 single-phase culverts of uneven sizes: one launch per step for all
+    (graph replay, autotuned block shape) against launching the instances one
+    at a time as the reference's loop does (one launch + device synchronise
+    per step, src/cuda/runCuda.cu:473-490)."""
+    from flowlb_amd import cases, core
+    insts = cases.batch_instances(args.instances)
+    built = []
+    for c in insts:
+        flags, rho0, u0 = c.build()
+        built.append(dict(nx=c.nx, ny=c.ny, tau=c.tau, flags=flags, rho0=rho0, u0=u0))
+    ncells = sum(c.N for c in insts)
+    with core.Batch(built) as b:
+        bx, grid, tune_ms = b.autotune(steps=6)
+        b.step(args.warmup)
+        b.sync()
+        ms = b.step_timed(args.steps)
+        info = b.info()
+    batched_ms = ms / args.steps
+    # one at a time: every instance on its own handle, per-step launch + sync
+    per_sync, per_async = 0.0, 0.0
+    k = args.single_steps
+    for c, d in zip(insts, built):
+        with core.Solver(d["flags"], c.nx, c.ny, c.options(), rho0=d["rho0"], u0=d["u0"]) as s:
+            s.step(2)
+            s.sync()
+            t = time.perf_counter()
+            for _ in range(k):
+                s.step(1)
+                s.sync()
+            per_sync += (time.perf_counter() - t) / k
+            t = time.perf_counter()
+            s.step(k)
+            s.sync()
+            per_async += (time.perf_counter() - t) / k
+    out = {"metric": METRIC, "value": round(ncells / (batched_ms * 1e-3) / 1e6, 1), "unit": "MLUPS",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(batched_ms, 5), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"batch_{args.instances}", "instances": args.instances,
+                      "cells": ncells, "autotuned_block_x": bx, "autotuned_grid": grid,
+                      "graph_chunk": info["graph_chunk"], "tiles": info["tiles"]},
+           "one_at_a_time_ms_per_step": round(per_sync * 1e3, 4),
+           "one_at_a_time_async_ms_per_step": round(per_async * 1e3, 4),
+           "speedup_vs_one_at_a_time": round(per_sync * 1e3 / batched_ms, 3),
+           "speedup_vs_one_at_a_time_async": round(per_async * 1e3 / batched_ms, 3),
+           "achieved_GBs": round(ncells * 73 / (batched_ms * 1e-3) / 1e9, 1)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,12 +248,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--fd-cells", type=int, default=16)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--single-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the workload is one domain on one MI355X; "
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload.startswith("adjoint"):
+        return bench_adjoint(args)
+    if args.workload.startswith("batch"):
+        return bench_batch(args)
     from flowlb_amd import cases
     case = cases.CASES[args.workload]
     r = measure(case, args.storage, args.steps, args.warmup, args.profile_steps)

@@ -52,6 +52,17 @@ class Culvert(C.Structure):
                 ("u_inlet", C.c_float), ("rho_noise", C.c_double), ("seed", C.c_uint64)]
 
 
+class AdjointSpec(C.Structure):
+    _fields_ = [("nsteps", C.c_uint64), ("t0", C.c_uint64), ("t1", C.c_uint64),
+                ("column", C.c_uint32), ("ckpt", C.c_uint32)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("tau", C.c_float), ("fx", C.c_float),
+                ("fy", C.c_float), ("use_force", C.c_int32), ("flags", C.POINTER(C.c_uint8)),
+                ("rho0", C.POINTER(C.c_float)), ("u0", C.POINTER(C.c_float))]
+
+
 class UnitsIn(C.Structure):
     _fields_ = [("length_si", C.c_double), ("nx", C.c_uint32), ("ref_velocity_si", C.c_double),
                 ("ref_velocity_lb", C.c_double), ("density_si", C.c_double),
@@ -95,6 +106,24 @@ PROTOTYPES = [
     ("flb_last_error", C.c_char_p, [_H]),
     ("flb_status_string", C.c_char_p, [C.c_int]),
     ("flb_destroy", None, [_H]),
+    ("flb_set_brinkman", C.c_int, [_H, _F]),
+    ("flb_objective", C.c_int, [_H, C.POINTER(AdjointSpec), _D]),
+    ("flb_adjoint", C.c_int, [_H, C.POINTER(AdjointSpec), _D, _F, _D, _F]),
+    ("flb_batch_create", C.c_int, [C.POINTER(Instance), C.c_uint32, C.c_int32, C.c_int32,
+                                   C.POINTER(_H)]),
+    ("flb_batch_configure", C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32]),
+    ("flb_batch_autotune", C.c_int, [_H, C.c_uint32, C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32), _F]),
+    ("flb_batch_step", C.c_int, [_H, C.c_uint64]),
+    ("flb_batch_sync", C.c_int, [_H]),
+    ("flb_batch_step_timed", C.c_int, [_H, C.c_uint64, _F]),
+    ("flb_batch_read", C.c_int, [_H, C.c_uint32, _F, _F]),
+    ("flb_batch_read_populations", C.c_int, [_H, C.c_uint32, _F]),
+    ("flb_batch_time", C.c_uint64, [_H]),
+    ("flb_batch_info", C.c_int, [_H, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]),
+    ("flb_batch_last_error", C.c_char_p, [_H]),
+    ("flb_batch_destroy", None, [_H]),
     ("flb_build_channel_flags", C.c_int, [C.POINTER(Channel), _U8]),
     ("flb_seed_free_surface", C.c_int, [_U8, C.c_uint32, C.c_uint32]),
     ("flb_init_free_surface_fields", C.c_int, [_U8, C.c_size_t, _F, _F, _F]),
@@ -251,9 +280,121 @@ class Solver:
               "flb_device_fields")
         return [p.value for p in ptrs]
 
+    # -- adjoint
+    def set_brinkman(self, alpha):
+        alpha = _f32(alpha, self.N)
+        check(lib().flb_set_brinkman(self.h, fptr(alpha)), self.h, "flb_set_brinkman")
+
+    def objective(self, nsteps, column, t0, t1, ckpt=100):
+        spec = AdjointSpec(nsteps, t0, t1, column, ckpt)
+        j = C.c_double()
+        check(lib().flb_objective(self.h, C.byref(spec), C.byref(j)), self.h, "flb_objective")
+        return j.value
+
+    def adjoint(self, nsteps, column, t0, t1, ckpt=100):
+        """Returns J, dJ/dalpha (N floats), dJ/du_inlet and the phase times (ms)."""
+        spec = AdjointSpec(nsteps, t0, t1, column, ckpt)
+        j = C.c_double()
+        duin = C.c_double()
+        g = np.zeros(self.N, np.float32)
+        ph = (C.c_float * 3)()
+        check(lib().flb_adjoint(self.h, C.byref(spec), C.byref(j), fptr(g), C.byref(duin), ph),
+              self.h, "flb_adjoint")
+        return j.value, g, duin.value, list(ph)
+
     def close(self):
         if getattr(self, "h", None):
             lib().flb_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many single-phase culverts advanced by one launch per step (flb_batch_*)."""
+
+    def __init__(self, instances, storage="fp32", device=0):
+        """instances: list of dicts with nx, ny, tau, flags, rho0, u0 (and fx,
+        fy, use_force)."""
+        self._keep = []
+        arr = (Instance * len(instances))()
+        self.sizes = []
+        for k, d in enumerate(instances):
+            fl = np.ascontiguousarray(d["flags"], dtype=np.uint8)
+            r0 = _f32(d.get("rho0"))
+            u0 = _f32(d.get("u0"))
+            self._keep += [fl, r0, u0]
+            arr[k] = Instance(d["nx"], d["ny"], d["tau"], d.get("fx", 0.0), d.get("fy", 0.0),
+                              int(d.get("use_force", 0)), u8ptr(fl), fptr(r0), fptr(u0))
+            self.sizes.append((d["nx"], d["ny"]))
+        h = C.c_void_p()
+        st = lib().flb_batch_create(arr, len(instances), STORAGE[storage] if isinstance(storage, str)
+                                    else storage, device, C.byref(h))
+        if st:
+            raise FlbError(st, "flb_batch_create: " + lib().flb_status_string(st).decode())
+        self.h = h
+
+    def _check(self, st, what):
+        if st:
+            raise FlbError(st, f"{what}: {lib().flb_status_string(st).decode()}: "
+                               f"{lib().flb_batch_last_error(self.h).decode()}")
+
+    def configure(self, block_x=256, grid=2048, graph_chunk=100):
+        self._check(lib().flb_batch_configure(self.h, block_x, grid, graph_chunk), "configure")
+
+    def autotune(self, steps=20):
+        b, g, ms = C.c_int32(), C.c_int32(), C.c_float()
+        self._check(lib().flb_batch_autotune(self.h, steps, C.byref(b), C.byref(g), C.byref(ms)),
+                    "autotune")
+        return b.value, g.value, ms.value
+
+    def step(self, n=1):
+        self._check(lib().flb_batch_step(self.h, n), "step")
+
+    def sync(self):
+        self._check(lib().flb_batch_sync(self.h), "sync")
+
+    def step_timed(self, n):
+        ms = C.c_float()
+        self._check(lib().flb_batch_step_timed(self.h, n, C.byref(ms)), "step_timed")
+        return ms.value
+
+    def read(self, i):
+        nx, ny = self.sizes[i]
+        u = np.empty(2 * nx * ny, np.float32)
+        rho = np.empty(nx * ny, np.float32)
+        self._check(lib().flb_batch_read(self.h, i, fptr(u), fptr(rho)), "read")
+        return dict(u=u, rho=rho)
+
+    def populations(self, i):
+        nx, ny = self.sizes[i]
+        f = np.empty(9 * nx * ny, np.float32)
+        self._check(lib().flb_batch_read_populations(self.h, i, fptr(f)), "read_populations")
+        return f
+
+    def info(self):
+        b, g, c, t = C.c_int32(), C.c_int32(), C.c_int32(), C.c_uint32()
+        self._check(lib().flb_batch_info(self.h, C.byref(b), C.byref(g), C.byref(c), C.byref(t)),
+                    "info")
+        return dict(block_x=b.value, grid=g.value, graph_chunk=c.value, tiles=t.value)
+
+    @property
+    def time(self):
+        return lib().flb_batch_time(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().flb_batch_destroy(self.h)
             self.h = None
 
     def __enter__(self):

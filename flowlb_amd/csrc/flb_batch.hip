@@ -1,0 +1,437 @@
+// flb_batch.hip -- many independent single-phase culverts of uneven sizes in
+// one kernel launch per step.  No reference counterpart: FlowLB runs one domain
+// per process.
+//
+// Every instance keeps the single-domain layout (nine population planes, flags,
+// fields, publication buffers) at its own offset of shared allocations.  The
+// work of a step is a packed list of tiles -- (instance, row, x-tile) -- over
+// all instances; a grid of a few blocks per CU walks the list (grid-stride),
+// so small and large instances share the machine and one launch covers them
+// all.  Runs of steps are replayed from hipGraphs captured for an even number
+// of steps, which removes the per-launch overhead.  A block-size/grid-size
+// autotuner times the candidates on the actual instance set.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "flb_cell.h"
+#include "flb_internal.h"
+
+namespace flb {
+
+struct Tile {
+  uint32_t inst;
+  uint32_t row;
+  uint32_t tx;
+  uint32_t pad;
+};
+
+template <class S, int ODD, bool MOM>
+__global__ void __launch_bounds__(1024) k_batch_step(const Tile* __restrict__ tiles, uint32_t ntiles,
+                                                      const Params* __restrict__ inst) {
+  for (uint32_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
+    const Tile t = tiles[k];
+    const Params& p = inst[t.inst];
+    const uint32_t x = t.tx * blockDim.x + threadIdx.x;
+    if (x < p.nx) {
+      const Cell c{x, t.row, x + t.row * p.nx, true};
+      stream_collide_cell<S, ODD, false, MOM, false>(p, c);
+    }
+  }
+}
+
+template <class S>
+__global__ void k_batch_to_f32(const Params* inst, uint32_t i, float* out) {
+  const Params& p = inst[i];
+  for (uint64_t n = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; n < p.N;
+       n += uint64_t(gridDim.x) * blockDim.x) {
+    const typename S::T* f = static_cast<const typename S::T*>(p.f);
+    for (int q = 0; q < 9; q++) out[q * p.N + n] = S::load(f + q * p.Np, n) + wgt(q);
+  }
+}
+
+template <class S>
+__global__ void k_batch_from_f32(const Params* inst, uint32_t i, const float* in) {
+  const Params& p = inst[i];
+  for (uint64_t n = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; n < p.N;
+       n += uint64_t(gridDim.x) * blockDim.x) {
+    typename S::T* f = static_cast<typename S::T*>(p.f);
+    for (int q = 0; q < 9; q++) S::store(f + q * p.Np, n, in[q * p.N + n]);
+  }
+}
+
+using BatchFn = void (*)(const Tile*, uint32_t, const Params*);
+template <class S>
+BatchFn batch_fn(int odd, bool mom) {
+  if (mom) return odd ? k_batch_step<S, 1, true> : k_batch_step<S, 0, true>;
+  return odd ? k_batch_step<S, 1, false> : k_batch_step<S, 0, false>;
+}
+BatchFn get_batch(int storage, int odd, bool mom) {
+  switch (storage) {
+    case 1: return batch_fn<StoreF16S>(odd, mom);
+    case 2: return batch_fn<StoreBF16S>(odd, mom);
+    default: return batch_fn<StoreF32>(odd, mom);
+  }
+}
+
+}  // namespace flb
+
+struct flb_batch {
+  int storage = 0;
+  int block_x = 256;
+  int grid = 0;            // blocks of the grid-stride launch
+  int graph_chunk = 100;   // steps per captured graph (even), 0 = no graphs
+  int ncu = 256;
+  hipStream_t stream = nullptr;
+  std::vector<flb::Params> host_inst;
+  flb::Params* inst = nullptr;
+  flb::Tile* tiles = nullptr;
+  uint32_t ntiles = 0;
+  std::vector<void*> owned;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
+  uint64_t t = 0;
+  std::string err;
+};
+
+namespace {
+
+#define BAT_HIP(b, call)                                                                   \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      (b)->err = std::string(#call) + ": " + hipGetErrorString(e_);                        \
+      return FLB_ERR_DEVICE;                                                               \
+    }                                                                                      \
+  } while (0)
+
+template <class T>
+int balloc(flb_batch* b, T** p, size_t count) {
+  void* v = nullptr;
+  if (hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    b->err = "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed";
+    return FLB_ERR_MEMORY;
+  }
+  b->owned.push_back(v);
+  *p = static_cast<T*>(v);
+  return FLB_OK;
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Tile list for block size bx, instances interleaved row by row so that
+// consecutive tiles of the grid-stride walk come from different instances.
+int build_tiles(flb_batch* b, int bx) {
+  std::vector<flb::Tile> tl;
+  uint32_t max_rows = 0;
+  for (const auto& p : b->host_inst) max_rows = std::max(max_rows, p.ny);
+  for (uint32_t r = 0; r < max_rows; ++r)
+    for (uint32_t i = 0; i < b->host_inst.size(); ++i) {
+      const auto& p = b->host_inst[i];
+      if (r >= p.ny) continue;
+      const uint32_t tx = (p.nx + bx - 1) / bx;
+      for (uint32_t k = 0; k < tx; ++k) tl.push_back(flb::Tile{i, r, k, 0});
+    }
+  if (b->tiles) {
+    (void)hipFree(b->tiles);
+    b->owned.erase(std::remove(b->owned.begin(), b->owned.end(), static_cast<void*>(b->tiles)),
+                   b->owned.end());
+    b->tiles = nullptr;
+  }
+  int rc = balloc(b, &b->tiles, tl.size());
+  if (rc) return rc;
+  BAT_HIP(b, hipMemcpy(b->tiles, tl.data(), tl.size() * sizeof(flb::Tile), hipMemcpyHostToDevice));
+  b->ntiles = uint32_t(tl.size());
+  b->block_x = bx;
+  return FLB_OK;
+}
+
+void drop_graph(flb_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->graph_steps = 0;
+}
+
+int launch_step(flb_batch* b, int odd, bool mom) {
+  hipLaunchKernelGGL(flb::get_batch(b->storage, odd, mom), dim3(b->grid), dim3(b->block_x), 0,
+                     b->stream, b->tiles, b->ntiles, b->inst);
+  BAT_HIP(b, hipGetLastError());
+  return FLB_OK;
+}
+
+// Graph of graph_chunk steps starting at an even t (parities alternate).
+int ensure_graph(flb_batch* b) {
+  if (b->graph || b->graph_chunk <= 0) return FLB_OK;
+  hipGraph_t g = nullptr;
+  BAT_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = FLB_OK;
+  for (int s = 0; s < b->graph_chunk && !rc; ++s) rc = launch_step(b, s & 1, false);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  BAT_HIP(b, e);
+  e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  BAT_HIP(b, e);
+  b->graph_steps = b->graph_chunk;
+  return FLB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* flb_batch_last_error(const flb_batch* b) { return b ? b->err.c_str() : ""; }
+
+void flb_batch_destroy(flb_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  for (void* v : b->owned) (void)hipFree(v);
+  delete b;
+}
+
+int flb_batch_create(const flb_instance* in, uint32_t count, int32_t storage, int32_t device,
+                     flb_batch** out) {
+  if (!out) return FLB_ERR_ARGUMENT;
+  *out = nullptr;
+  if (!in || count == 0 || storage < 0 || storage > 2) return FLB_ERR_ARGUMENT;
+  for (uint32_t i = 0; i < count; ++i) {
+    const flb_instance& s = in[i];
+    if (!s.flags || s.nx < 2 || s.ny < 2 || !(s.tau > 0.5f)) return FLB_ERR_ARGUMENT;
+    const size_t N = size_t(s.nx) * s.ny;
+    if (s.flags[N - 1] & FLB_INLET) return FLB_ERR_ARGUMENT;
+    if ((s.flags[0] & FLB_OUTLET) && !(s.flags[0] & FLB_INLET)) return FLB_ERR_ARGUMENT;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return FLB_ERR_DEVICE;
+  }
+  if (device < 0 || device >= ndev) return FLB_ERR_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return FLB_ERR_DEVICE;
+  flb_batch* b = new flb_batch();
+  b->storage = storage;
+  auto bail = [&](int rc) {
+    flb_batch_destroy(b);
+    return rc;
+  };
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) b->ncu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(FLB_ERR_DEVICE);
+  const size_t eb = storage == 0 ? 4 : 2;
+  // Offsets of every instance in the shared allocations (elements).
+  std::vector<size_t> off_f(count), off_c(count);
+  size_t tot_f = 0, tot_c = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const size_t N = size_t(in[i].nx) * in[i].ny;
+    const size_t Np = align_up(N, 64);
+    off_f[i] = tot_f;
+    tot_f += 9 * Np;
+    off_c[i] = tot_c;
+    tot_c += align_up(N, 64);
+  }
+  uint8_t* f = nullptr;
+  uint8_t* flags = nullptr;
+  float *rho = nullptr, *u = nullptr, *pr0 = nullptr, *pr1 = nullptr, *pu0 = nullptr, *pu1 = nullptr;
+  int rc;
+  if ((rc = balloc(b, &f, tot_f * eb)) || (rc = balloc(b, &flags, tot_c)) ||
+      (rc = balloc(b, &rho, tot_c)) || (rc = balloc(b, &u, 2 * tot_c)) ||
+      (rc = balloc(b, &pr0, tot_c)) || (rc = balloc(b, &pr1, tot_c)) ||
+      (rc = balloc(b, &pu0, 2 * tot_c)) || (rc = balloc(b, &pu1, 2 * tot_c)) ||
+      (rc = balloc(b, &b->inst, count)))
+    return bail(rc);
+  float* staging = nullptr;
+  size_t max_n = 0;
+  for (uint32_t i = 0; i < count; ++i) max_n = std::max(max_n, size_t(in[i].nx) * in[i].ny);
+  if (hipMalloc(&staging, 9 * max_n * sizeof(float)) != hipSuccess) return bail(FLB_ERR_MEMORY);
+  b->host_inst.resize(count);
+  std::vector<float> fs, hr, hu;
+  for (uint32_t i = 0; i < count && !rc; ++i) {
+    const flb_instance& s = in[i];
+    const size_t N = size_t(s.nx) * s.ny;
+    flb::Params& p = b->host_inst[i];
+    p = flb::Params{};
+    p.nx = s.nx;
+    p.ny = s.ny;
+    p.N = N;
+    p.Np = align_up(N, 64);
+    p.inv_tau = 1.0f / s.tau;
+    p.use_force = s.use_force;
+    p.fx = s.use_force ? s.fx : 0.0f;
+    p.fy = s.use_force ? s.fy : 0.0f;
+    p.f = f + off_f[i] * eb;
+    p.flags = flags + off_c[i];
+    p.rho = rho + off_c[i];
+    p.u = u + 2 * off_c[i];
+    p.pub_rho[0] = pr0 + off_c[i];
+    p.pub_rho[1] = pr1 + off_c[i];
+    p.pub_u[0] = pu0 + 2 * off_c[i];
+    p.pub_u[1] = pu1 + 2 * off_c[i];
+    hr.assign(N, 1.0f);
+    hu.assign(2 * N, 0.0f);
+    if (s.rho0) std::memcpy(hr.data(), s.rho0, N * sizeof(float));
+    if (s.u0) std::memcpy(hu.data(), s.u0, 2 * N * sizeof(float));
+    flb::host_initial_populations(s.nx, s.ny, s.rho0, s.u0, fs);
+    hipError_t e = hipMemcpy(p.flags, s.flags, N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p.rho, hr.data(), N * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p.u, hu.data(), 2 * N * 4, hipMemcpyHostToDevice);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+      e = hipMemcpy(p.pub_rho[k], hr.data(), N * 4, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(p.pub_u[k], hu.data(), 2 * N * 4, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemcpy(staging, fs.data(), 9 * N * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->inst + i, &p, sizeof(p), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      void (*conv)(const flb::Params*, uint32_t, const float*) =
+          storage == 1 ? flb::k_batch_from_f32<flb::StoreF16S>
+                       : (storage == 2 ? flb::k_batch_from_f32<flb::StoreBF16S>
+                                       : flb::k_batch_from_f32<flb::StoreF32>);
+      hipLaunchKernelGGL(conv, dim3(512), dim3(256), 0, b->stream, b->inst, i, staging);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    }
+    if (e != hipSuccess) {
+      b->err = std::string("batch upload: ") + hipGetErrorString(e);
+      rc = FLB_ERR_DEVICE;
+    }
+  }
+  (void)hipFree(staging);
+  if (rc) return bail(rc);
+  b->grid = 8 * b->ncu;
+  if ((rc = build_tiles(b, 256))) return bail(rc);
+  *out = b;
+  return FLB_OK;
+}
+
+int flb_batch_configure(flb_batch* b, int32_t block_x, int32_t grid, int32_t graph_chunk) {
+  if (!b) return FLB_ERR_ARGUMENT;
+  if (block_x % 64 || block_x < 64 || block_x > 1024 || grid < 1 || graph_chunk < 0 ||
+      graph_chunk % 2)
+    return FLB_ERR_ARGUMENT;
+  drop_graph(b);
+  b->grid = grid;
+  b->graph_chunk = graph_chunk;
+  if (block_x != b->block_x) return build_tiles(b, block_x);
+  return FLB_OK;
+}
+
+int flb_batch_step(flb_batch* b, uint64_t nsteps) {
+  if (!b) return FLB_ERR_ARGUMENT;
+  int rc = FLB_OK;
+  uint64_t left = nsteps;
+  while (left > 0 && !rc) {
+    // Graph replays need an even start; the last step writes the readback fields.
+    if (b->graph_chunk > 0 && (b->t & 1) == 0 && left > uint64_t(b->graph_chunk)) {
+      if ((rc = ensure_graph(b))) break;
+      BAT_HIP(b, hipGraphLaunch(b->graph, b->stream));
+      b->t += uint64_t(b->graph_steps);
+      left -= uint64_t(b->graph_steps);
+    } else {
+      rc = launch_step(b, int(b->t & 1), left == 1);
+      b->t += 1;
+      left -= 1;
+    }
+  }
+  return rc;
+}
+
+int flb_batch_sync(flb_batch* b) {
+  if (!b) return FLB_ERR_ARGUMENT;
+  BAT_HIP(b, hipStreamSynchronize(b->stream));
+  return FLB_OK;
+}
+
+int flb_batch_step_timed(flb_batch* b, uint64_t nsteps, float* ms) {
+  if (!b) return FLB_ERR_ARGUMENT;
+  hipEvent_t e0, e1;
+  BAT_HIP(b, hipEventCreate(&e0));
+  BAT_HIP(b, hipEventCreate(&e1));
+  BAT_HIP(b, hipEventRecord(e0, b->stream));
+  int rc = flb_batch_step(b, nsteps);
+  BAT_HIP(b, hipEventRecord(e1, b->stream));
+  BAT_HIP(b, hipEventSynchronize(e1));
+  float v = 0.0f;
+  BAT_HIP(b, hipEventElapsedTime(&v, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = v;
+  return rc;
+}
+
+// Times block sizes {64,128,256,512} x grids {2,4,8,16} blocks per CU over
+// `steps` steps each (the state advances) and keeps the fastest.
+int flb_batch_autotune(flb_batch* b, uint32_t steps, int32_t* best_block, int32_t* best_grid,
+                       float* best_ms) {
+  if (!b || steps == 0) return FLB_ERR_ARGUMENT;
+  const int blocks[4] = {64, 128, 256, 512};
+  const int per_cu[4] = {2, 4, 8, 16};
+  const int chunk = b->graph_chunk;
+  float best = 1e30f;
+  int bb = b->block_x, bg = b->grid;
+  int rc = FLB_OK;
+  for (int bx : blocks)
+    for (int pc : per_cu) {
+      if ((rc = flb_batch_configure(b, bx, pc * b->ncu, 0))) return rc;
+      if ((rc = flb_batch_step(b, 2))) return rc;  // warm
+      float ms = 0.0f;
+      if ((rc = flb_batch_step_timed(b, steps, &ms))) return rc;
+      if (ms < best) {
+        best = ms;
+        bb = bx;
+        bg = pc * b->ncu;
+      }
+    }
+  if ((rc = flb_batch_configure(b, bb, bg, chunk))) return rc;
+  if (best_block) *best_block = bb;
+  if (best_grid) *best_grid = bg;
+  if (best_ms) *best_ms = best / float(steps);
+  return FLB_OK;
+}
+
+int flb_batch_read(flb_batch* b, uint32_t i, float* u, float* rho) {
+  if (!b || i >= b->host_inst.size()) return FLB_ERR_ARGUMENT;
+  const flb::Params& p = b->host_inst[i];
+  BAT_HIP(b, hipStreamSynchronize(b->stream));
+  if (u) BAT_HIP(b, hipMemcpy(u, p.u, 2 * p.N * sizeof(float), hipMemcpyDeviceToHost));
+  if (rho) BAT_HIP(b, hipMemcpy(rho, p.rho, p.N * sizeof(float), hipMemcpyDeviceToHost));
+  return FLB_OK;
+}
+
+int flb_batch_read_populations(flb_batch* b, uint32_t i, float* f) {
+  if (!b || !f || i >= b->host_inst.size()) return FLB_ERR_ARGUMENT;
+  const flb::Params& p = b->host_inst[i];
+  float* tmp = nullptr;
+  BAT_HIP(b, hipMalloc(&tmp, 9 * p.N * sizeof(float)));
+  void (*conv)(const flb::Params*, uint32_t, float*) =
+      b->storage == 1 ? flb::k_batch_to_f32<flb::StoreF16S>
+                      : (b->storage == 2 ? flb::k_batch_to_f32<flb::StoreBF16S>
+                                         : flb::k_batch_to_f32<flb::StoreF32>);
+  hipLaunchKernelGGL(conv, dim3(512), dim3(256), 0, b->stream, b->inst, i, tmp);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e == hipSuccess) e = hipMemcpy(f, tmp, 9 * p.N * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(tmp);
+  BAT_HIP(b, e);
+  return FLB_OK;
+}
+
+uint64_t flb_batch_time(const flb_batch* b) { return b ? b->t : 0; }
+
+int flb_batch_info(const flb_batch* b, int32_t* block_x, int32_t* grid, int32_t* graph_chunk,
+                   uint32_t* ntiles) {
+  if (!b) return FLB_ERR_ARGUMENT;
+  if (block_x) *block_x = b->block_x;
+  if (grid) *grid = b->grid;
+  if (graph_chunk) *graph_chunk = b->graph_chunk;
+  if (ntiles) *ntiles = b->ntiles;
+  return FLB_OK;
+}
+
+}  // extern "C"

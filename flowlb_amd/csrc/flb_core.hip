@@ -18,49 +18,7 @@
 #include "../../include/flb_core.h"
 #include "flb_device.h"
 
-namespace flb {
-using KernelFn = void (*)(const Params);
-using MassFn = void (*)(const Params, double*);
-using ConvOutFn = void (*)(const Params, float*);
-using ConvInFn = void (*)(const Params, const float*);
-using HistFn = void (*)(const Params, unsigned long long*);
-KernelFn get_stream_collide(int storage, int odd, bool fs, bool mom);
-KernelFn get_fs1(int storage, int odd);
-KernelFn get_fs2();
-KernelFn get_fs3(int storage, int odd);
-KernelFn get_fs4();
-MassFn get_pop_mass(int storage);
-MassFn get_fs_mass();
-ConvOutFn get_pop_to_f32(int storage);
-ConvInFn get_f32_to_pop(int storage);
-HistFn get_flag_hist();
-}  // namespace flb
-
-struct flb_handle {
-  flb_options opt{};
-  uint32_t nx = 0, ny = 0;
-  uint64_t N = 0, Np = 0;
-  int elem_bytes = 4;
-  hipStream_t stream = nullptr;
-  void* f = nullptr;
-  uint8_t* flags[2] = {nullptr, nullptr};
-  int cur = 0;
-  float* rho = nullptr;
-  float* u = nullptr;
-  float* pub_rho[2] = {nullptr, nullptr};
-  float* pub_u[2] = {nullptr, nullptr};
-  float* mass = nullptr;
-  float* excess = nullptr;
-  float* phi = nullptr;
-  float* rec = nullptr;
-  float* alpha = nullptr;
-  double* partial = nullptr;
-  size_t partial_len = 0;
-  uint64_t t = 0;
-  float inv_tau = 0.0f;
-  std::vector<void*> owned;
-  std::string err;
-};
+#include "flb_internal.h"
 
 namespace {
 
@@ -116,6 +74,7 @@ flb::Params params(const flb_handle* h, int bx) {
   p.phi = h->phi;
   p.rec = h->rec;
   p.alpha = h->alpha;
+  p.trace = nullptr;
   return p;
 }
 
@@ -302,6 +261,8 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
     if ((rc = dev_alloc(h, &h->phi, N))) return bail(rc);
     if ((rc = dev_alloc(h, &h->rec, 9 * N))) return bail(rc);
   }
+  if ((rc = dev_alloc(h, &h->rho_init, N))) return bail(rc);
+  if ((rc = dev_alloc(h, &h->u_init, 2 * N))) return bail(rc);
   const flb::Params pu = params(h, kUtilBlock);
   h->partial_len = pu.nblocks;
   if ((rc = dev_alloc(h, &h->partial, h->partial_len))) return bail(rc);
@@ -315,6 +276,8 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   if (fs && (rc = upload(h, h->flags[1], flags, N))) return bail(rc);
   if ((rc = upload(h, h->rho, hr.data(), N * sizeof(float)))) return bail(rc);
   if ((rc = upload(h, h->u, hu.data(), 2 * N * sizeof(float)))) return bail(rc);
+  if ((rc = upload(h, h->rho_init, hr.data(), N * sizeof(float)))) return bail(rc);
+  if ((rc = upload(h, h->u_init, hu.data(), 2 * N * sizeof(float)))) return bail(rc);
   for (int k = 0; k < 2; ++k) {
     if ((rc = upload(h, h->pub_rho[k], hr.data(), N * sizeof(float)))) return bail(rc);
     if ((rc = upload(h, h->pub_u[k], hu.data(), 2 * N * sizeof(float)))) return bail(rc);
@@ -391,6 +354,23 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
   h->t += 1;
   return FLB_OK;
 }
+
+}  // extern "C"
+
+namespace flb {
+void host_initial_populations(uint32_t nx, uint32_t ny, const float* rho0, const float* u0,
+                              std::vector<float>& fs) {
+  initial_populations(nx, ny, rho0, u0, fs);
+}
+int set_error(flb_handle* h, int code, const std::string& msg) { return fail(h, code, msg); }
+Params make_params(const flb_handle* h, int bx) { return params(h, bx); }
+int handle_block_x(const flb_handle* h) { return block_x(h); }
+int enqueue_step_ex(flb_handle* h, bool moments, hipEvent_t* ev) {
+  return enqueue_step(h, moments, ev);
+}
+}  // namespace flb
+
+extern "C" {
 
 int flb_step(flb_handle* h, uint64_t nsteps) {
   if (!h) return FLB_ERR_ARGUMENT;

@@ -80,6 +80,7 @@ struct Params {
   float* phi;
   float* rec;            // reconstructed gas populations, 9 planes of N
   const float* alpha;    // Brinkman solid fraction or nullptr
+  float* trace;          // adjoint trace of one step: 3 planes of N (or nullptr)
 };
 
 // Block -> tile mapping.  Workgroups are dispatched round-robin over the 8

@@ -1,0 +1,68 @@
+// flb_internal.h -- handle layout and helpers shared by the translation units
+// of libflbcore.so (not part of the public ABI).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/flb_core.h"
+#include "flb_device.h"
+
+namespace flb {
+using KernelFn = void (*)(const Params);
+using MassFn = void (*)(const Params, double*);
+using ConvOutFn = void (*)(const Params, float*);
+using ConvInFn = void (*)(const Params, const float*);
+using HistFn = void (*)(const Params, unsigned long long*);
+KernelFn get_stream_collide(int storage, int odd, bool fs, bool mom);
+KernelFn get_fs1(int storage, int odd);
+KernelFn get_fs2();
+KernelFn get_fs3(int storage, int odd);
+KernelFn get_fs4();
+KernelFn get_stream_collide_trace(int odd, bool mom);
+MassFn get_pop_mass(int storage);
+MassFn get_fs_mass();
+ConvOutFn get_pop_to_f32(int storage);
+ConvInFn get_f32_to_pop(int storage);
+HistFn get_flag_hist();
+}  // namespace flb
+
+struct flb_handle {
+  flb_options opt{};
+  uint32_t nx = 0, ny = 0;
+  uint64_t N = 0, Np = 0;
+  int elem_bytes = 4;
+  hipStream_t stream = nullptr;
+  void* f = nullptr;
+  uint8_t* flags[2] = {nullptr, nullptr};
+  int cur = 0;
+  float* rho = nullptr;
+  float* u = nullptr;
+  float* pub_rho[2] = {nullptr, nullptr};
+  float* pub_u[2] = {nullptr, nullptr};
+  float* mass = nullptr;
+  float* excess = nullptr;
+  float* phi = nullptr;
+  float* rec = nullptr;
+  float* alpha = nullptr;     // Brinkman solid fraction (allocated on first use)
+  float* rho_init = nullptr;  // rho0 and u0 as given to flb_create
+  float* u_init = nullptr;
+  double* partial = nullptr;
+  size_t partial_len = 0;
+  uint64_t t = 0;
+  float inv_tau = 0.0f;
+  std::vector<void*> owned;
+  std::string err;
+};
+
+
+namespace flb {
+// Helpers defined in flb_core.hip.
+int set_error(flb_handle* h, int code, const std::string& msg);
+Params make_params(const flb_handle* h, int bx);
+int handle_block_x(const flb_handle* h);
+int enqueue_step_ex(flb_handle* h, bool moments, hipEvent_t* ev);
+// Shifted initial populations feq(rho0, u0) in the odd-step slot pattern.
+void host_initial_populations(uint32_t nx, uint32_t ny, const float* rho0, const float* u0,
+                              std::vector<float>& fs);
+}  // namespace flb

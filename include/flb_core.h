@@ -159,6 +159,75 @@ const char* flb_status_string(int status);
 /* Frees every device buffer of the handle (src/cuda/runCuda.cu:111-127). */
 void flb_destroy(flb_handle* h);
 
+/* ------------------------------------------------------------ adjoint */
+/* Gradients of a time-averaged discharge.  No reference counterpart (FlowLB
+   has no adjoint); these entry points extend the solver for optimisation of a
+   culvert inlet.  Monofluid, SRT, FP32 storage, no subgrid model. */
+
+/* Brinkman drag: per-cell solid fraction alpha (N floats) adds the force
+   -alpha*rho*u to the Guo forcing of non-inlet/outlet cells, with the half-force
+   velocity shift solved for u.  NULL removes it. */
+int flb_set_brinkman(flb_handle* h, const float* alpha);
+
+/* Objective J = (1/(t1-t0+1)) * sum_{t=t0..t1} Q(t), with Q(t) the sum over the
+   non-wall cells of `column` of rho*u_x at step t, over a run of nsteps steps
+   from the handle's initial state (t = 0).  Checkpoints every ckpt steps. */
+typedef struct flb_adjoint_spec {
+  uint64_t nsteps;
+  uint64_t t0, t1;
+  uint32_t column;
+  uint32_t ckpt;
+} flb_adjoint_spec;
+
+/* Forward objective only; the handle is left at t = 0. */
+int flb_objective(flb_handle* h, const flb_adjoint_spec* spec, double* J);
+
+/* Forward sweep with checkpoints, reverse sweep with recomputation of each
+   checkpoint segment.  Outputs J, dJ/dalpha (N floats, NULL allowed; zeros
+   without Brinkman drag), dJ/du_inlet (x component of the velocity held by
+   the inlet cells).  phase_ms (NULL allowed) receives the device time of the
+   forward sweep, the recompute passes and the reverse sweep.  The handle is
+   left at t = 0. */
+int flb_adjoint(flb_handle* h, const flb_adjoint_spec* spec, double* J, float* dj_dalpha,
+                double* dj_duin, float* phase_ms);
+
+/* ------------------------------------------------------------ batch */
+/* Many independent monofluid SRT domains of uneven sizes advanced together, one
+   kernel launch per step for all of them, runs of steps replayed from HIP
+   graphs.  No reference counterpart (FlowLB runs one domain per process, each
+   step being launch + device synchronisation, src/cuda/runCuda.cu:473-490). */
+typedef struct flb_instance {
+  uint32_t nx, ny;
+  float tau, fx, fy;
+  int32_t use_force;
+  const uint8_t* flags;  /* N bytes */
+  const float* rho0;     /* N floats or NULL */
+  const float* u0;       /* 2N floats or NULL (inlet velocity at inlet cells) */
+} flb_instance;
+
+typedef struct flb_batch flb_batch;
+
+int flb_batch_create(const flb_instance* instances, uint32_t count, int32_t storage,
+                     int32_t device, flb_batch** out);
+/* Threads per block (multiple of 64), blocks of the grid-stride launch, steps
+   per captured graph (even; 0 = plain launches). */
+int flb_batch_configure(flb_batch* b, int32_t block_x, int32_t grid, int32_t graph_chunk);
+/* Times block sizes x grid sizes over `steps` steps each (the state advances)
+   and keeps the fastest. */
+int flb_batch_autotune(flb_batch* b, uint32_t steps, int32_t* best_block, int32_t* best_grid,
+                       float* best_ms_per_step);
+int flb_batch_step(flb_batch* b, uint64_t nsteps);
+int flb_batch_sync(flb_batch* b);
+int flb_batch_step_timed(flb_batch* b, uint64_t nsteps, float* ms);
+/* Readback of one instance (fields of the last step; raw FP32 populations). */
+int flb_batch_read(flb_batch* b, uint32_t instance, float* u, float* rho);
+int flb_batch_read_populations(flb_batch* b, uint32_t instance, float* f);
+uint64_t flb_batch_time(const flb_batch* b);
+int flb_batch_info(const flb_batch* b, int32_t* block_x, int32_t* grid, int32_t* graph_chunk,
+                   uint32_t* ntiles);
+const char* flb_batch_last_error(const flb_batch* b);
+void flb_batch_destroy(flb_batch* b);
+
 /* ------------------------------------------------------------ host set-up */
 
 /* Rectangle in SI units: centre (x0, y0), height (y), width (x)

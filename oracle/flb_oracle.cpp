@@ -382,7 +382,12 @@ struct Sim {
         loadf(f.data(), N, n, nb, tt, in);
         out[0] = in[0];
         load_outgoing(f.data(), N, n, nb, tt, out);
-        T m = mass[n];
+        // As written the increments are added to the mass one by one; the
+        // conservative mode sums them first and adds the total once, so that
+        // excess shares far below the rounding step of the mass are not lost.
+        T m0 = mass[n];
+        T dm = 0.0f;
+        T& m = p.mass_mode == 0 ? m0 : dm;
         for (int i = 1; i < 9; i++) m += excess[nb[i]];
         const uint8_t e = flag & FLUID_GAS_INTERFACE;
         if (e == FLUID && p.mass_mode == 0) {
@@ -415,7 +420,7 @@ struct Sim {
           for (int i = 1; i < 9; i++) nphi[i] = phi[nb[i]];
           // As written the own fill level is the fresh one; the conservative
           // mode uses the stored one so that both cells of a link agree.
-          nphi[0] = p.mass_mode == 0 ? phi_of(m, r) : phi[n];
+          nphi[0] = p.mass_mode == 0 ? phi_of(m0, r) : phi[n];
           const T rhol2 = 0.5f / r;
           ux = ux + rhol2 * p.fx;
           uy = uy + rhol2 * p.fy;
@@ -450,7 +455,8 @@ struct Sim {
           }
           rec_mask[n] = mask;
         }
-        mass[n] = m;
+        if (p.mass_mode != 0) m0 = m0 + dm;
+        mass[n] = m0;
       }
     }
   }
@@ -828,6 +834,257 @@ double orc_discharge(void* s, uint32_t column) {
     acc += q->rho[n] * q->u[n];
   }
   return acc;
+}
+
+// ---------------------------------------------------------------- adjoint
+// Discrete adjoint of the single-phase step (SRT, Guo forcing, Brinkman drag,
+// inlet/outlet, implicit bounce-back), FP64.  Extension without reference
+// counterpart; pinned by central finite differences in the tests.
+//
+// Each cell reads and writes one slot set per parity, so the reverse step is
+// per cell and in place: the adjoints of the populations a cell stored are
+// read from its store slots, the adjoints of the populations it loaded are
+// written to its load slots.  Inlet and outlet cells depend on the previous
+// step's published density / velocity of a neighbour; those adjoints travel
+// backwards through rbar / ubar (two parity buffers, like the forward).
+
+namespace {
+
+struct Snapshot {
+  std::vector<double> f, rho, u;
+};
+
+// Store slot of population i of cell n at parity odd (storef).
+inline size_t store_slot(size_t N, uint32_t n, const uint32_t* nb, bool odd, int i) {
+  if (i == 0) return n;
+  if (i & 1) return nb[i] + N * (odd ? i + 1 : i);
+  return n + N * (odd ? i - 1 : i);
+}
+// Load slot of population i of cell n at parity odd (loadf).
+inline size_t load_slot(size_t N, uint32_t n, const uint32_t* nb, bool odd, int i) {
+  if (i == 0) return n;
+  if (i & 1) return n + N * (odd ? i : i + 1);
+  return nb[i - 1] + N * (odd ? i : i - 1);
+}
+
+struct Adjoint {
+  Sim<double>& s;
+  uint32_t col;
+  uint64_t t0, t1;
+  double qbar;
+  std::vector<double> abar, rbar[2], ubar[2], galpha;
+  double duin = 0.0;
+
+  Adjoint(Sim<double>& sim, uint32_t c, uint64_t a, uint64_t b)
+      : s(sim), col(c), t0(a), t1(b), qbar(1.0 / double(b - a + 1)) {
+    abar.assign(9 * s.N, 0.0);
+    for (int k = 0; k < 2; k++) {
+      rbar[k].assign(s.N, 0.0);
+      ubar[k].assign(2 * s.N, 0.0);
+    }
+    galpha.assign(s.N, 0.0);
+  }
+
+  static void feq_and_grad(double rho, double ux, double uy, double* fe, double* dfx, double* dfy) {
+    const double* w = Weights<double>::w;
+    const double usq = ux * ux + uy * uy;
+    for (int i = 0; i < 9; i++) {
+      const double cu = CX[i] * ux + CY[i] * uy;
+      fe[i] = w[i] * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+      dfx[i] = w[i] * rho * (3.0 * CX[i] + 9.0 * cu * CX[i] - 3.0 * ux);
+      dfy[i] = w[i] * rho * (3.0 * CY[i] + 9.0 * cu * CY[i] - 3.0 * uy);
+    }
+  }
+
+  // Reverse of step t.  st: state before step t (populations and fields).
+  void backward(uint64_t t, const Snapshot& st) {
+    const size_t N = s.N;
+    const bool odd = t & 1u;
+    const int cur = int(t & 1u), prev = cur ^ 1;
+    const bool in_window = t >= t0 && t <= t1;
+    const double k3 = double(0.33333334f);
+    const double om = double(s.p.inv_tau);
+    const double tl = double(1.0f - 0.5f * s.p.inv_tau);
+    const double Fx = s.p.use_force ? double(s.p.fx) : 0.0, Fy = s.p.use_force ? double(s.p.fy) : 0.0;
+    const double* w = Weights<double>::w;
+    for (uint32_t y = 0; y < s.ny; y++)
+      for (uint32_t x = 0; x < s.nx; x++) {
+        const uint32_t n = x + y * s.nx;
+        const uint8_t fl = s.flags[n];
+        if (fl & (GAS | WALL)) continue;
+        uint32_t nb[9];
+        neighbours(x, y, s.nx, s.ny, nb);
+        double v[9];
+        for (int i = 0; i < 9; i++) v[i] = abar[store_slot(N, n, nb, odd, i)];
+        const bool pub_r = n > 0 && (s.flags[n - 1] & INLET);
+        const bool pub_v = n + 1 < N && (s.flags[n + 1] & OUTLET) && !(fl & INLET);
+        double rb_pub = 0.0, ubx_pub = 0.0, uby_pub = 0.0;
+        if (pub_r) {
+          rb_pub = rbar[cur][n];
+          rbar[cur][n] = 0.0;
+        }
+        if (pub_v) {
+          ubx_pub = ubar[cur][n];
+          uby_pub = ubar[cur][N + n];
+          ubar[cur][n] = ubar[cur][N + n] = 0.0;
+        }
+        const uint8_t io = fl & (INLET | OUTLET);
+        double gb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (io) {
+          double rho, ux, uy;
+          if (io & INLET) {
+            rho = st.rho[n + 1 < N ? n + 1 : n];
+            ux = st.u[n];
+            uy = st.u[N + n];
+          } else {
+            rho = st.rho[n];
+            ux = st.u[n - 1];
+            uy = st.u[N + n - 1];
+          }
+          const double uxt = ux + 0.5 / rho * Fx, uyt = uy + 0.5 / rho * Fy;
+          double fe[9], dfx[9], dfy[9];
+          feq_and_grad(rho, uxt, uyt, fe, dfx, dfy);
+          double rbt = rb_pub, ubx = ubx_pub, uby = uby_pub;
+          for (int i = 0; i < 9; i++) {
+            rbt += v[i] * fe[i] / rho;
+            ubx += v[i] * dfx[i];
+            uby += v[i] * dfy[i];
+          }
+          // u = u_read + F/(2 rho)
+          rbt += -(ubx * Fx + uby * Fy) * 0.5 / (rho * rho);
+          if (io & INLET) {
+            if (n + 1 < N) rbar[prev][n + 1] += rbt;
+            duin += ubx;
+          } else {
+            ubar[prev][n - 1] += ubx;
+            ubar[prev][N + n - 1] += uby;
+          }
+        } else {
+          double g[9];
+          loadf(st.f.data(), N, n, nb, t, g);
+          double rho = 0.0, mx = 0.0, my = 0.0;
+          for (int i = 0; i < 9; i++) {
+            rho += g[i];
+            mx += CX[i] * g[i];
+            my += CY[i] * g[i];
+          }
+          const double a = s.alpha.empty() ? 0.0 : s.alpha[n];
+          const double sa = 1.0 / (1.0 + 0.5 * a);
+          const double ux = (mx + 0.5 * Fx) * sa / rho, uy = (my + 0.5 * Fy) * sa / rho;
+          const double Gx = Fx - a * rho * ux, Gy = Fy - a * rho * uy;
+          const bool forced = s.p.use_force || !s.alpha.empty();
+          double fe[9], dfx[9], dfy[9];
+          feq_and_grad(rho, ux, uy, fe, dfx, dfy);
+          double rb = rb_pub, ubx = ubx_pub, uby = uby_pub, Gbx = 0.0, Gby = 0.0;
+          for (int i = 0; i < 9; i++) {
+            gb[i] = (1.0 - om) * v[i];
+            const double feb = om * v[i];
+            rb += feb * fe[i] / rho;
+            ubx += feb * dfx[i];
+            uby += feb * dfy[i];
+            if (forced) {
+              const double Fb = tl * v[i];
+              const double cG = CX[i] * Gx + CY[i] * Gy;
+              const double cu = CX[i] * ux + CY[i] * uy;
+              ubx += Fb * 9.0 * w[i] * (cG * CX[i] - k3 * Gx);
+              uby += Fb * 9.0 * w[i] * (cG * CY[i] - k3 * Gy);
+              Gbx += Fb * 9.0 * w[i] * (CX[i] * (cu + k3) - k3 * ux);
+              Gby += Fb * 9.0 * w[i] * (CY[i] * (cu + k3) - k3 * uy);
+            }
+          }
+          // G = F - a rho u
+          ubx += -a * rho * Gbx;
+          uby += -a * rho * Gby;
+          rb += -a * (ux * Gbx + uy * Gby);
+          double ab = -rho * (ux * Gbx + uy * Gby);
+          if (in_window && x == col) {  // objective: qbar * rho * ux
+            rb += qbar * ux;
+            ubx += qbar * rho;
+          }
+          // u = (m + F/2) sa / rho
+          const double mbx = ubx * sa / rho, mby = uby * sa / rho;
+          const double ubu = ubx * ux + uby * uy;
+          rb += -ubu / rho;
+          ab += -0.5 * sa * ubu;
+          for (int i = 0; i < 9; i++) gb[i] += rb + CX[i] * mbx + CY[i] * mby;
+          if (!s.alpha.empty()) galpha[n] += ab;
+        }
+        for (int i = 0; i < 9; i++) abar[load_slot(N, n, nb, odd, i)] = gb[i];
+      }
+  }
+};
+
+Snapshot snap(const Sim<double>& s) { return Snapshot{s.f, s.rho, s.u}; }
+void restore(Sim<double>& s, const Snapshot& z, uint64_t t) {
+  s.f = z.f;
+  s.rho = z.rho;
+  s.u = z.u;
+  s.t = t;
+}
+
+}  // namespace
+
+double orc_objective(void* sp, uint64_t nsteps, uint32_t column, uint64_t t0, uint64_t t1) {
+  auto& s = *static_cast<Sim<double>*>(sp);
+  const Snapshot z0 = snap(s);
+  const uint64_t tstart = s.t;
+  double J = 0.0;
+  for (uint64_t k = 0; k < nsteps; k++) {
+    s.step();
+    if (k >= t0 && k <= t1) J += orc_discharge(sp, column);
+  }
+  restore(s, z0, tstart);
+  return J / double(t1 - t0 + 1);
+}
+
+int orc_adjoint(void* sp, uint64_t nsteps, uint32_t column, uint64_t t0, uint64_t t1,
+                uint32_t ckpt, double* dj_dalpha, double* dj_duin, double* j_out) {
+  auto& s = *static_cast<Sim<double>*>(sp);
+  if (s.p.free_surface || s.p.collision != 0 || s.p.smagorinsky) return 1;
+  if (t1 < t0 || t1 >= nsteps || ckpt == 0 || column >= s.nx) return 2;
+  const uint64_t tstart = s.t;
+  if (tstart != 0) return 3;
+  const Snapshot z0 = snap(s);
+  // forward with checkpoints every ckpt steps
+  std::vector<Snapshot> cps;
+  double J = 0.0;
+  for (uint64_t k = 0; k < nsteps; k++) {
+    if (k % ckpt == 0) cps.push_back(snap(s));
+    s.step();
+    if (k >= t0 && k <= t1) J += orc_discharge(sp, column);
+  }
+  J /= double(t1 - t0 + 1);
+  Adjoint adj(s, column, t0, t1);
+  std::vector<Snapshot> seg;
+  for (int64_t c = int64_t(cps.size()) - 1; c >= 0; c--) {
+    const uint64_t ts = uint64_t(c) * ckpt;
+    const uint64_t te = std::min<uint64_t>(ts + ckpt, nsteps);
+    restore(s, cps[size_t(c)], ts);
+    seg.clear();
+    for (uint64_t k = ts; k < te; k++) {
+      seg.push_back(snap(s));
+      s.step();
+    }
+    for (uint64_t k = te; k-- > ts;) adj.backward(k, seg[size_t(k - ts)]);
+  }
+  // The initial populations feq(rho0, u0) of inlet cells depend on u_inlet.
+  const size_t N = s.N;
+  for (uint32_t y = 0; y < s.ny; y++)
+    for (uint32_t x = 0; x < s.nx; x++) {
+      const uint32_t n = x + y * s.nx;
+      if (!(s.flags[n] & INLET) || (s.flags[n] & (GAS | WALL))) continue;
+      uint32_t nb[9];
+      neighbours(x, y, s.nx, s.ny, nb);
+      double fe[9], dfx[9], dfy[9];
+      Adjoint::feq_and_grad(z0.rho[n], z0.u[n], z0.u[N + n], fe, dfx, dfy);
+      for (int i = 0; i < 9; i++) adj.duin += adj.abar[store_slot(N, n, nb, true, i)] * dfx[i];
+    }
+  restore(s, z0, tstart);
+  if (dj_dalpha)
+    for (size_t n = 0; n < N; n++) dj_dalpha[n] = adj.galpha[n];
+  if (dj_duin) *dj_duin = adj.duin;
+  if (j_out) *j_out = J;
+  return 0;
 }
 
 // ---------------------------------------------------------------- FP32 API

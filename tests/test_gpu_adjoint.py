@@ -1,0 +1,121 @@
+"""GPU adjoint (flb_adjoint) against the FP64 oracle adjoint and central finite
+differences of the FP64 oracle objective.  Bars: J within 1e-6 relative of
+the oracle; dJ/dalpha over the Brinkman window and dJ/du_inlet within 1e-4
+relative (L2 over the window) of the FP64 adjoint, which itself agrees with
+finite differences to 1e-6 (tests/test_oracle_adjoint.py)."""
+import numpy as np
+import pytest
+
+from flowlb_amd import cases, core
+
+pytestmark = pytest.mark.gpu
+
+STEPS, T0, T1 = 240, 160, 239
+
+
+def small_case():
+    c = cases.scaled(cases.ADJOINT, 8)          # 128 x 32
+    flags, rho0, u0 = c.build()
+    rng = np.random.default_rng(7)
+    alpha = np.zeros(c.N)
+    win = (40, 47, 1, 8)
+    for y in range(win[2], win[3] + 1):
+        for x in range(win[0], win[1] + 1):
+            alpha[x + y * c.nx] = rng.uniform(0, 1)
+    return c, flags, rho0, u0, alpha, win
+
+
+def gpu_solver(c, flags, rho0, u0, alpha):
+    s = core.Solver(flags, c.nx, c.ny, c.options(), rho0=rho0, u0=u0)
+    s.set_brinkman(alpha.astype(np.float32))
+    return s
+
+
+def test_adjoint_matches_fp64_adjoint(orc):
+    c, flags, rho0, u0, alpha, win = small_case()
+    col = c.nx - 2
+    with gpu_solver(c, flags, rho0, u0, alpha) as s:
+        J, g, duin, ph = s.adjoint(STEPS, col, T0, T1, ckpt=50)
+    o = orc.Oracle(flags, c.nx, c.ny, c.tau, use_force=True, fx=c.force[0], fy=c.force[1],
+                   rho0=rho0.astype(np.float64), u0=u0.astype(np.float64),
+                   alpha=alpha.astype(np.float32).astype(np.float64), threads=8)
+    Jr, gr, duinr = o.adjoint(STEPS, col, T0, T1, ckpt=50)
+    m = alpha != 0
+    eg = np.linalg.norm(g[m] - gr[m]) / np.linalg.norm(gr[m])
+    eall = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    print(f"\nadjoint 128x32: J {J:.8e} vs {Jr:.8e} (rel {abs(J - Jr) / abs(Jr):.2e}); "
+          f"dJ/dalpha relL2 window {eg:.2e} all {eall:.2e}; dJ/du_in {duin:.6e} vs {duinr:.6e} "
+          f"(rel {abs(duin - duinr) / abs(duinr):.2e}); phases ms {ph}")
+    assert abs(J - Jr) <= 1e-6 * abs(Jr)
+    assert eg <= 1e-4
+    assert abs(duin - duinr) <= 1e-4 * abs(duinr)
+
+
+def test_checkpoint_interval_does_not_change_result():
+    c, flags, rho0, u0, alpha, _ = small_case()
+    out = []
+    for ck in (7, 50, 1000):
+        with gpu_solver(c, flags, rho0, u0, alpha) as s:
+            out.append(s.adjoint(STEPS, c.nx - 2, T0, T1, ckpt=ck))
+    for J, g, duin, _ in out[1:]:
+        assert J == out[0][0]
+        np.testing.assert_array_equal(g, out[0][1])
+        assert duin == out[0][2]
+
+
+def test_adjoint_leaves_initial_state_and_objective_agrees():
+    c, flags, rho0, u0, alpha, _ = small_case()
+    with gpu_solver(c, flags, rho0, u0, alpha) as s:
+        f0 = s.populations()
+        J1 = s.objective(STEPS, c.nx - 2, T0, T1)
+        J2, _, _, _ = s.adjoint(STEPS, c.nx - 2, T0, T1, ckpt=30)
+        assert s.time == 0
+        np.testing.assert_array_equal(s.populations(), f0)
+        assert J1 == J2
+        s.step(STEPS)   # and stepping still works afterwards
+
+
+def test_unsupported_configurations_rejected():
+    c, flags, rho0, u0, alpha, _ = small_case()
+    with core.Solver(flags, c.nx, c.ny, c.options(storage="fp16s"), rho0=rho0, u0=u0) as s:
+        with pytest.raises(core.FlbError) as e:
+            s.adjoint(10, c.nx - 2, 0, 9)
+        assert e.value.status == 5
+    with gpu_solver(c, flags, rho0, u0, alpha) as s:
+        s.step(3)
+        with pytest.raises(core.FlbError):
+            s.adjoint(10, c.nx - 2, 0, 9)
+
+
+def test_torch_autograd_wrapper(orc):
+    """The autograd Function returns the adjoint gradients; they match central
+    finite differences of the FP64 oracle objective."""
+    torch = pytest.importorskip("torch")
+    from adjoint_autograd import CulvertDischarge
+    c, flags, rho0, u0, alpha, win = small_case()
+    x0, x1, y0, y1 = win
+    aw = torch.tensor(alpha.reshape(c.ny, c.nx)[y0:y1 + 1, x0:x1 + 1], dtype=torch.float64,
+                      requires_grad=True)
+    uin = torch.tensor(float(np.float32(c.u_inlet)), dtype=torch.float64, requires_grad=True)
+    J = CulvertDischarge.apply(aw, uin, c, win, STEPS, T0, T1, 50)
+    J.backward()
+    a32 = alpha.astype(np.float32).astype(np.float64)
+
+    def J_of(a, u):
+        uu = u0.astype(np.float64).copy()
+        uu[:c.N][(flags & core.INLET) != 0] = u
+        return orc.Oracle(flags, c.nx, c.ny, c.tau, use_force=True, fx=c.force[0], fy=c.force[1],
+                          rho0=rho0.astype(np.float64), u0=uu, alpha=a, threads=8).objective(
+            STEPS, c.nx - 2, T0, T1)
+    h = 1e-4
+    for (x, y) in [(41, 2), (44, 5), (47, 8)]:
+        n = x + y * c.nx
+        ap, am = a32.copy(), a32.copy()
+        ap[n] += h
+        am[n] -= h
+        fd = (J_of(ap, float(uin)) - J_of(am, float(uin))) / (2 * h)
+        gt = float(aw.grad[y - y0, x - x0])
+        assert abs(gt - fd) <= 1e-4 * abs(fd) + 1e-9, (x, y, gt, fd)
+    hu = 1e-6
+    fd = (J_of(a32, float(uin) + hu) - J_of(a32, float(uin) - hu)) / (2 * hu)
+    assert abs(float(uin.grad) - fd) <= 1e-4 * abs(fd), (float(uin.grad), fd)

@@ -6,6 +6,8 @@ few hundred steps (a threshold test such as mass > 1.001 rho can fall on
 either side under FP32 rounding); mass, fill level and velocity within the
 stated absolute bounds; total mass conserved to 1e-7 relative per 10^4 steps
 in the conservative mode."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -14,6 +16,8 @@ from flowlb_amd import cases, core
 pytestmark = pytest.mark.gpu
 
 SMALL = cases.scaled(cases.FS_4096, 16)   # 256 x 64 dam break behind a culvert
+# Same geometry, stronger gravity: the front moves within a few hundred steps.
+FAST = dataclasses.replace(SMALL, name="dam_break_256x64_fast", force=(0.0, -2e-4), tau=0.6)
 
 
 def oracle_for(orc, case, flags, **kw):
@@ -46,16 +50,16 @@ def test_first_steps_exact_flags(orc, mass_mode, gas_check_corrected):
 @pytest.mark.parametrize("mass_mode", [0, 1])
 def test_dam_break_parity(orc, mass_mode):
     steps = 400
-    g, r, flags0 = run_both(orc, SMALL, steps, mass_mode)
+    g, r, flags0 = run_both(orc, FAST, steps, mass_mode)
     same = (g["flags"] == r["flags"]).mean()
     moved = (r["flags"] != flags0).sum()
     du = np.abs(g["u"] - r["u"]).max()
     dm = np.abs(g["mass"] - r["mass"]).max()
     print(f"\nfs 256x64 x {steps} mode {mass_mode}: flags equal {same:.5f} (cells changed {moved}) "
           f"max|du| {du:.2e} max|dm| {dm:.2e}")
-    assert moved > 0
+    assert moved > 100
     assert same >= 0.999
-    assert du < 5e-3 and dm < 5e-2
+    assert du < 1e-3 and dm < 1e-2
 
 
 def test_conservation_fp32_1e4_steps():
