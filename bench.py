@@ -237,6 +237,21 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def copy_probe(args):
+    """A plain device copy of known size (torch, 1 GiB read + 1 GiB written per
+    launch): the reference point that scales the FETCH_SIZE / WRITE_SIZE
+    counters of the other kernels to bytes."""
+    import torch
+    n = (1 << 30) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
+                      "launches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +272,8 @@ def main():
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "copy_probe":
+        return copy_probe(args)
     if args.workload.startswith("adjoint"):
         return bench_adjoint(args)
     if args.workload.startswith("batch"):

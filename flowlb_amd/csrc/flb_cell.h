@@ -76,6 +76,25 @@ __device__ __forceinline__ void neighbours(const Params& p, uint32_t x, uint32_t
   nb[8] = xl + yu;
 }
 
+// A non-inlet, non-outlet fluid cell whose eight neighbours are fluid or
+// interface (also while interface neighbours turn interface->fluid or ->gas
+// inside stream-collide).  For such a cell the mass exchange of free-surface
+// pass 1 over all links, sum_i (in_opp(i) - out_i), equals rho_in(t) -
+// rho*(t-1): the density of the incoming populations minus the sum of the
+// populations the cell emitted, and its mass before the pass is the density it
+// had, so the updated mass is rho_in(t) plus the excess shares it pulls (up to
+// the collision's rounding and Guo residual, ~1e-8 per step).
+__device__ __forceinline__ bool interior_fluid(const Params& p, uint8_t fl, const uint32_t* nb) {
+  if ((fl & (kFGI | kInlet | kOutlet)) != kFluid) return false;
+  bool all = true;
+#pragma unroll
+  for (int i = 1; i < 9; i++) {
+    const uint8_t e = p.flags[nb[i]] & kFGI;
+    all = all && (e == kFluid || e == kInterface || e == kIF || e == kIG);
+  }
+  return all;
+}
+
 // loadf (src/cuda/cellCalculations.cuh:185-196) with ODD = t % 2.
 template <class S, int ODD>
 __device__ __forceinline__ void load_incoming(const Params& p, const uint32_t* nb, float* g) {
@@ -189,6 +208,12 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
       p.trace[N + n] = mx;
       p.trace[2 * N + n] = my;
     }
+  }
+  if (FS && p.mass_mode != 0 && interior_fluid(p, fl, nb)) {
+    double m = double(rho);
+#pragma unroll
+    for (int i = 1; i < 9; i++) m += double(p.excess[nb[i]]);
+    p.mass[n] = float(m);
   }
   if (FS && (fl & kInterface) == kInterface) {  // interface -> fluid / gas test (:353-367)
     const float m = p.mass[n];

@@ -347,7 +347,7 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
     if ((rc = run(FLB_K_FS2, flb::get_fs2()))) return rc;
     if ((rc = run(FLB_K_FS3, flb::get_fs3(st, odd)))) return rc;
     if ((rc = run(FLB_K_FS4, flb::get_fs4()))) return rc;
-    h->cur ^= 1;
+    if (h->opt.mass_mode == FLB_MASS_AS_WRITTEN) h->cur ^= 1;  // pass 4 wrote the other buffer
   } else {
     if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, false, moments)))) return rc;
   }

@@ -41,6 +41,9 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
   if (fl & (kGas | kWall | kInlet)) return;
   uint32_t nb[9];
   neighbours(p, c.x, c.y, nb);
+  // Conservative mode: the mass of a fluid cell whose eight neighbours are all
+  // fluid or interface is updated by stream-collide instead (interior_fluid).
+  if (p.mass_mode != 0 && interior_fluid(p, fl, nb)) return;
   float in[9], out[9];
   load_incoming<S, ODD>(p, nb, in);
   out[0] = in[0];
@@ -205,31 +208,42 @@ __global__ void __launch_bounds__(1024) k_fs4(const Params p) {
   const Cell c = locate(p);
   if (!c.valid) return;
   const uint32_t n = c.n;
+  // As written the receivers are counted on the flags before this pass, so the
+  // new flags go to the second buffer.  In the conservative mode the count
+  // uses the settled flag of each neighbour, which is the same before and
+  // after its own transition: flags are updated in place.
+  const bool cons = p.mass_mode != 0;
   const uint8_t fl = p.flags[n];
   if (fl & (kWall | kMovingWall | kInlet)) {
-    p.flags_next[n] = fl;
+    if (!cons) p.flags_next[n] = fl;
     return;
   }
   const uint8_t e = fl & (kFGI | kGasInterface);
   float m = p.mass[n];
   if (e == kGas && m == 0.0f) {
-    // Empty gas cell: only a share left from the previous step needs clearing.
-    p.flags_next[n] = fl;
-    if (p.excess[n] != 0.0f) p.excess[n] = 0.0f;
+    // Empty gas cell.  A cell that turned gas in the previous step left an
+    // excess share for its neighbours and marked its mass -0.0; clear both.
+    if (!cons) p.flags_next[n] = fl;
+    if (signbit(m)) {
+      p.mass[n] = 0.0f;
+      p.excess[n] = 0.0f;
+    }
     return;
   }
   const float rho = p.rho[n];
   float ex = 0.0f, ph = 0.0f;
+  bool write_phi = true;
   uint8_t nfl = fl;
   switch (e) {
     case kFluid:
       ex = m - rho;
       m = rho;
       ph = 1.0f;
+      write_phi = false;  // a cell that is fluid here was fluid after the last pass
       break;
     case kGas:
       ex = m;
-      m = 0.0f;
+      m = -0.0f;
       break;
     case kIF:
       nfl = fl & uint8_t(~kInterface);
@@ -240,7 +254,7 @@ __global__ void __launch_bounds__(1024) k_fs4(const Params p) {
     case kIG:
       nfl = fl & uint8_t(~kInterface);
       ex = m;
-      m = 0.0f;
+      m = -0.0f;
       break;
     case kInterface:
     case kGasInterface:
@@ -263,7 +277,7 @@ __global__ void __launch_bounds__(1024) k_fs4(const Params p) {
 #pragma unroll
   for (int i = 1; i < 9; i++) {
     const uint8_t q = p.flags[nb[i]];
-    if (p.mass_mode == 0) {
+    if (!cons) {
       const uint8_t r = q & (kIF | kGasInterface);
       cont += int(r == kFluid || r == kInterface || r == kIF || r == kGasInterface);
     } else {
@@ -277,8 +291,9 @@ __global__ void __launch_bounds__(1024) k_fs4(const Params p) {
   }
   p.mass[n] = m;
   p.excess[n] = ex;
-  p.phi[n] = ph;
-  p.flags_next[n] = nfl;
+  if (write_phi) p.phi[n] = ph;
+  if (!cons) p.flags_next[n] = nfl;
+  else if (nfl != fl) p.flags[n] = nfl;
 }
 
 // ------------------------------------------------------------- utilities
