@@ -278,7 +278,7 @@ def main():
         return bench_adjoint(args)
     if args.workload.startswith("batch"):
         return bench_batch(args)
-    from flowlb_amd import cases
+    from flowlb_amd import cases, core
     case = cases.CASES[args.workload]
     r = measure(case, args.storage, args.steps, args.warmup, args.profile_steps)
     ms_step = r["ms"] / args.steps
@@ -314,8 +314,7 @@ def main():
         "step_roofline": {"algorithmic_bytes_per_step": int(r["step_bytes"]),
                           "achieved_GBs": round(r["step_bytes"] / (ms_step * 1e-3) / 1e9, 1),
                           "frac": round(r["step_bytes"] / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "kernel_ms": {k: round(v, 5) for k, v in zip(
-            ["stream_collide", "fs1", "fs2", "fs3", "fs4"], r["per_kernel_ms"])},
+        "kernel_ms": {k: round(v, 5) for k, v in zip(core.KERNEL_NAMES, r["per_kernel_ms"])},
         "wall_s_timed_region": round(r["wall"], 4),
     }
     if not args.no_sweep:

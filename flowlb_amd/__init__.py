@@ -5,9 +5,10 @@ of include/flb_core.h); this package is its host-side binding.
 """
 from . import core
 from .core import (FLUID, GAS, INTERFACE, WALL, MOVING_WALL, INLET, OUTLET, GAS_INTERFACE,
-                   FlbError, Solver, build_channel_flags, build_culvert, default_options,
+                   MONOFLUID, FREE_SURFACE, SRT, TRT, Deck, FlbError, Solver,
+                   build_channel_flags, build_culvert, default_options,
                    init_free_surface_fields, seed_free_surface, units)
 
 __all__ = ["core", "FLUID", "GAS", "INTERFACE", "WALL", "MOVING_WALL", "INLET", "OUTLET",
-           "GAS_INTERFACE", "FlbError", "Solver", "build_channel_flags", "build_culvert",
+           "GAS_INTERFACE", "MONOFLUID", "FREE_SURFACE", "SRT", "TRT", "Deck", "FlbError", "Solver", "build_channel_flags", "build_culvert",
            "default_options", "init_free_surface_fields", "seed_free_surface", "units"]

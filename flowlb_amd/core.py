@@ -13,8 +13,10 @@ LIB_PATH = Path(__file__).resolve().parent / "libflbcore.so"
 FLUID, GAS, INTERFACE, WALL, MOVING_WALL, INLET, OUTLET, GAS_INTERFACE = (
     0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80)
 STORAGE = {"fp32": 0, "fp16s": 1, "bf16s": 2}
-K_STREAM_COLLIDE, K_FS1, K_FS2, K_FS3, K_FS4 = range(5)
-KERNEL_NAMES = ["stream_collide", "fs1", "fs2", "fs3", "fs4"]
+MONOFLUID, FREE_SURFACE = 0, 1     # flb_options.problem
+SRT, TRT = 0, 1                    # flb_options.collision
+K_STREAM_COLLIDE, K_FS1, K_FS_UPDATE = range(3)
+KERNEL_NAMES = ["stream_collide", "fs1", "fs_update"]
 
 
 class FlbError(RuntimeError):
@@ -63,6 +65,11 @@ class Instance(C.Structure):
                 ("rho0", C.POINTER(C.c_float)), ("u0", C.POINTER(C.c_float))]
 
 
+class VtiField(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("type", C.c_int32), ("components", C.c_uint32),
+                ("data", C.c_void_p), ("scale", C.c_double), ("flip_y", C.c_int32)]
+
+
 class UnitsIn(C.Structure):
     _fields_ = [("length_si", C.c_double), ("nx", C.c_uint32), ("ref_velocity_si", C.c_double),
                 ("ref_velocity_lb", C.c_double), ("density_si", C.c_double),
@@ -79,6 +86,26 @@ class UnitsOut(C.Structure):
                 ("length_scale", C.c_double), ("time_scale", C.c_double),
                 ("mass_scale", C.c_double)]
 
+
+class DeckInfo(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("x_min", C.c_double),
+                ("x_max", C.c_double), ("y_min", C.c_double), ("y_max", C.c_double),
+                ("size", C.c_double), ("n_points", C.c_uint32), ("n_cdws", C.c_uint32),
+                ("n_obstacles", C.c_uint32), ("n_initial", C.c_uint32), ("problem", C.c_int32),
+                ("bc_left", C.c_uint8), ("bc_right", C.c_uint8), ("bc_up", C.c_uint8),
+                ("bc_down", C.c_uint8), ("collision", C.c_int32), ("smagorinsky", C.c_int32),
+                ("surface_tension", C.c_int32), ("use_force", C.c_int32),
+                ("precision", C.c_int32), ("block_x", C.c_int32), ("block_y", C.c_int32),
+                ("block_z", C.c_int32), ("graphics", C.c_int32),
+                ("ref_velocity_lb", C.c_double), ("ref_velocity_si", C.c_double),
+                ("flow", C.c_double), ("relaxation_time", C.c_double),
+                ("kinematic_viscosity", C.c_double), ("density", C.c_double),
+                ("velocity_si", C.c_double * 3), ("acceleration", C.c_double * 3),
+                ("volumetric_force", C.c_double * 3), ("time_simulation", C.c_float),
+                ("time_save", C.c_float)]
+
+
+DECK_DOMAIN, DECK_CALCULATION = 0, 1
 
 # Exported symbols and their prototypes: (name, restype, argtypes).
 _H = C.c_void_p
@@ -124,11 +151,21 @@ PROTOTYPES = [
                                  C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]),
     ("flb_batch_last_error", C.c_char_p, [_H]),
     ("flb_batch_destroy", None, [_H]),
+    ("flb_write_vti", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                C.c_double, C.POINTER(VtiField), C.c_uint32, C.c_int32]),
+    ("flb_write_vti_series", C.c_int, [C.c_char_p, C.c_uint32, C.c_double]),
+    ("flb_save_vti", C.c_int, [_H, C.c_char_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_double,
+                               C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]),
     ("flb_build_channel_flags", C.c_int, [C.POINTER(Channel), _U8]),
     ("flb_seed_free_surface", C.c_int, [_U8, C.c_uint32, C.c_uint32]),
     ("flb_init_free_surface_fields", C.c_int, [_U8, C.c_size_t, _F, _F, _F]),
     ("flb_build_culvert", C.c_int, [C.POINTER(Culvert), _U8, _F, _F]),
     ("flb_units", C.c_int, [C.POINTER(UnitsIn), C.POINTER(UnitsOut)]),
+    ("flb_deck_load", C.c_int, [C.c_char_p, C.c_int, C.POINTER(_H), C.c_char_p, C.c_size_t]),
+    ("flb_deck_get_info", C.c_int, [_H, C.POINTER(DeckInfo)]),
+    ("flb_deck_build_flags", C.c_int, [_H, _U8]),
+    ("flb_deck_units_in", C.c_int, [_H, C.POINTER(UnitsIn)]),
+    ("flb_deck_free", None, [_H]),
 ]
 
 _lib = None
@@ -228,7 +265,7 @@ class Solver:
     def step_timed(self, n, per_kernel=False):
         tot = C.c_float()
         if per_kernel:
-            arr = (C.c_float * 5)()
+            arr = (C.c_float * len(KERNEL_NAMES))()
             check(lib().flb_step_timed(self.h, n, C.byref(tot), arr), self.h, "flb_step_timed")
             return tot.value, list(arr)
         check(lib().flb_step_timed(self.h, n, C.byref(tot), None), self.h, "flb_step_timed")
@@ -301,6 +338,12 @@ class Solver:
         check(lib().flb_adjoint(self.h, C.byref(spec), C.byref(j), fptr(g), C.byref(duin), ph),
               self.h, "flb_adjoint")
         return j.value, g, duin.value, list(ph)
+
+    def save_vti(self, path, spacing=1.0, origin=(0.0, 0.0), velocity_scale=1.0,
+                 density_scale=1.0, binary=False):
+        check(lib().flb_save_vti(self.h, str(path).encode(), self.nx, self.ny,
+                                 int(self.free_surface), origin[0], origin[1], spacing,
+                                 velocity_scale, density_scale, int(binary)), self.h, "flb_save_vti")
 
     def close(self):
         if getattr(self, "h", None):
@@ -452,8 +495,84 @@ def build_culvert(nx, ny, free_surface=False, bc=(INLET, OUTLET, WALL, WALL), em
     return fl, rho0, u0
 
 
+def write_vti(path, nx, ny, fields, origin=(0.0, 0.0), spacing=1.0, binary=False):
+    """fields: list of (name, array, scale, flip_y); uint8 arrays are written as
+    UInt8, float arrays as Float32, arrays of 2N values as 2-component vectors."""
+    keep = []
+    arr = (VtiField * max(1, len(fields)))()
+    N = nx * ny
+    for k, (name, data, scale, flip) in enumerate(fields):
+        if data.dtype == np.uint8:
+            d = np.ascontiguousarray(data)
+            t = 0
+        else:
+            d = np.ascontiguousarray(data, dtype=np.float32)
+            t = 1
+        comps = d.size // N
+        keep.append(d)
+        arr[k] = VtiField(name.encode(), t, comps, d.ctypes.data, scale, int(flip))
+    check(lib().flb_write_vti(str(path).encode(), nx, ny, origin[0], origin[1], spacing, arr,
+                              len(fields), int(binary)), None, "flb_write_vti")
+
+
+def write_vti_series(path, count, time_interval):
+    check(lib().flb_write_vti_series(str(path).encode(), count, time_interval), None,
+          "flb_write_vti_series")
+
+
 def units(**kw):
     i = UnitsIn(**kw)
     o = UnitsOut()
     check(lib().flb_units(C.byref(i), C.byref(o)), None, "flb_units")
     return o
+
+
+class Deck:
+    """A FlowLB case directory (Geometry.txt, OptionsCalc.txt, CSV outlines)
+    read by the native reader; `flags()` gives the domain's cell flags."""
+
+    def __init__(self, directory, mode=DECK_DOMAIN):
+        h = _H()
+        err = C.create_string_buffer(512)
+        st = lib().flb_deck_load(str(directory).encode(), mode, C.byref(h), err, len(err))
+        if st != 0:
+            raise FlbError(st, f"flb_deck_load: {lib().flb_status_string(st).decode()}: "
+                               f"{err.value.decode()}")
+        self.h = h
+        self.info = DeckInfo()
+        check(lib().flb_deck_get_info(self.h, C.byref(self.info)), None, "flb_deck_get_info")
+
+    @property
+    def nx(self):
+        return self.info.nx
+
+    @property
+    def ny(self):
+        return self.info.ny
+
+    def flags(self):
+        out = np.empty(self.info.nx * self.info.ny, np.uint8)
+        check(lib().flb_deck_build_flags(self.h, u8ptr(out)), None, "flb_deck_build_flags")
+        return out
+
+    def units_in(self):
+        i = UnitsIn()
+        check(lib().flb_deck_units_in(self.h, C.byref(i)), None, "flb_deck_units_in")
+        return i
+
+    def close(self):
+        if self.h:
+            lib().flb_deck_free(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -107,8 +107,10 @@ __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjP
       float v[9];
 #pragma unroll
       for (int i = 0; i < 9; i++) v[i] = a.abar[store_index<ODD>(p, n, nb, i)];
-      const bool pub_r = n > 0 && (p.flags[n - 1] & kInlet);
-      const bool pub_v = n + 1 < N && (p.flags[n + 1] & kOutlet) && !(fl & kInlet);
+      const uint8_t fl_prev = p.flags[n > 0 ? n - 1 : n];  // unconditional: one round trip
+      const uint8_t fl_next = p.flags[n + 1 < N ? n + 1 : n];
+      const bool pub_r = n > 0 && (fl_prev & kInlet);
+      const bool pub_v = n + 1 < N && (fl_next & kOutlet) && !(fl & kInlet);
       float rb = 0.0f, ubx = 0.0f, uby = 0.0f;
       if (pub_r) {
         rb = a.rbar[ODD][n];

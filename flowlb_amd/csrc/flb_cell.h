@@ -76,6 +76,13 @@ __device__ __forceinline__ void neighbours(const Params& p, uint32_t x, uint32_t
   nb[8] = xl + yu;
 }
 
+// Fluid/gas/interface bits of the eight neighbours, issued together so that
+// they cost one memory round trip.
+__device__ __forceinline__ void neighbour_flags(const Params& p, const uint32_t* nb, uint8_t* nf) {
+#pragma unroll
+  for (int i = 1; i < 9; i++) nf[i] = p.flags[nb[i]] & kFGI;
+}
+
 // A non-inlet, non-outlet fluid cell whose eight neighbours are fluid or
 // interface (also while interface neighbours turn interface->fluid or ->gas
 // inside stream-collide).  For such a cell the mass exchange of free-surface
@@ -84,12 +91,18 @@ __device__ __forceinline__ void neighbours(const Params& p, uint32_t x, uint32_t
 // populations the cell emitted, and its mass before the pass is the density it
 // had, so the updated mass is rho_in(t) plus the excess shares it pulls (up to
 // the collision's rounding and Guo residual, ~1e-8 per step).
-__device__ __forceinline__ bool interior_fluid(const Params& p, uint8_t fl, const uint32_t* nb) {
+//
+// In the conservative mode a fluid cell carries its mass between pass 1 and
+// pass 4 as the deviation mass - rho: stream-collide writes the pulled shares
+// of an interior cell, pass 1 the deviation of any other fluid cell, and pass 4
+// turns the deviation into the excess.  Shares far below the rounding step of
+// rho (~6e-8) are never rounded onto rho's grid, where they would be lost.
+__device__ __forceinline__ bool interior_fluid(uint8_t fl, const uint8_t* nf) {
   if ((fl & (kFGI | kInlet | kOutlet)) != kFluid) return false;
   bool all = true;
 #pragma unroll
   for (int i = 1; i < 9; i++) {
-    const uint8_t e = p.flags[nb[i]] & kFGI;
+    const uint8_t e = nf[i];
     all = all && (e == kFluid || e == kInterface || e == kIF || e == kIG);
   }
   return all;
@@ -143,11 +156,15 @@ template <class S, int ODD, bool FS, bool MOM, bool TRACE>
 __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell c) {
   const uint32_t n = c.n;
   const uint64_t N = p.N;
-  const uint8_t fl = p.flags[n];
   // Density of cell n is read next step by an inlet at n-1, velocity by an
-  // outlet at n+1 (lbmKernel2D.cu:336,344-345); both roles are static.
-  const bool pub_r = n > 0 && (p.flags[n - 1] & kInlet);
-  const bool pub_v = n + 1 < N && (p.flags[n + 1] & kOutlet) && !(fl & kInlet);
+  // outlet at n+1 (lbmKernel2D.cu:336,344-345); both roles are static.  The
+  // three flags are loaded unconditionally (clamped index, same cache line) so
+  // that they share one memory round trip instead of three branch-guarded ones.
+  const uint8_t fl = p.flags[n];
+  const uint8_t fl_prev = p.flags[n > 0 ? n - 1 : n];
+  const uint8_t fl_next = p.flags[n + 1 < N ? n + 1 : n];
+  const bool pub_r = n > 0 && (fl_prev & kInlet);
+  const bool pub_v = n + 1 < N && (fl_next & kOutlet) && !(fl & kInlet);
   if (fl & (kGas | kWall)) {
     if (FS) {  // a source cell that is gas this step keeps its last value
       if (pub_r) p.pub_rho[ODD][n] = p.pub_rho[ODD ^ 1][n];
@@ -162,14 +179,29 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   neighbours(p, c.x, c.y, nb);
   float g[9];
   load_incoming<S, ODD>(p, nb, g);
-  if (FS && (fl & kFGI) == kInterface) {
+  // Free surface: every load that depends only on the own flag is issued here,
+  // with the populations, so that the cell waits for memory once.
+  uint8_t nf[9];
+  float ex[9], rc[9], m_if = 0.0f;
+  const bool cons = FS && p.mass_mode != 0;
+  const bool fluid_dev = cons && (fl & (kFGI | kInlet | kOutlet)) == kFluid;
+  const bool iface = FS && (fl & kFGI) == kInterface;
+  if (FS) neighbour_flags(p, nb, nf);
+  if (fluid_dev) {
+#pragma unroll
+    for (int i = 1; i < 9; i++) ex[i] = p.excess[nb[i]];
+  }
+  if (iface) {
+    m_if = p.mass[n];
+#pragma unroll
+    for (int k = 1; k < 9; k++) rc[k] = p.rec[k * N + n];
     // Reconstructed gas populations of free-surface pass 1 replace incoming
     // ones (storeReconstructedGasf, cellCalculations.cuh:250-259).
 #pragma unroll
     for (int k = 1; k < 9; k++) {
       const int opp = (k & 1) ? k + 1 : k - 1;
-      const uint32_t gate = p.gas_check_corrected ? nb[opp] : nb[k];
-      if ((p.flags[gate] & kFGI) == kGas) g[k] = p.rec[k * N + n];
+      const uint8_t gate = p.gas_check_corrected ? nf[opp] : nf[k];
+      if (gate == kGas) g[k] = rc[k];
     }
   }
   const uint8_t io = fl & (kInlet | kOutlet);
@@ -209,23 +241,21 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
       p.trace[2 * N + n] = my;
     }
   }
-  if (FS && p.mass_mode != 0 && interior_fluid(p, fl, nb)) {
-    double m = double(rho);
+  if (fluid_dev && interior_fluid(fl, nf)) {  // deviation = pulled shares
+    double m = 0.0;
 #pragma unroll
-    for (int i = 1; i < 9; i++) m += double(p.excess[nb[i]]);
+    for (int i = 1; i < 9; i++) m += double(ex[i]);
     p.mass[n] = float(m);
   }
-  if (FS && (fl & kInterface) == kInterface) {  // interface -> fluid / gas test (:353-367)
-    const float m = p.mass[n];
+  if (iface) {  // interface -> fluid / gas test (:353-367)
     bool not_fluid = true, not_gas = true;
 #pragma unroll
     for (int i = 1; i < 9; i++) {
-      const uint8_t e = p.flags[nb[i]] & kFGI;
-      not_fluid = not_fluid && (e != kFluid);
-      not_gas = not_gas && (e != kGas);
+      not_fluid = not_fluid && (nf[i] != kFluid);
+      not_gas = not_gas && (nf[i] != kGas);
     }
-    if ((m > 1.001f * rho) || not_gas) p.flags[n] = (fl & uint8_t(~kFGI)) | kIF;
-    else if ((m < -0.001f) || not_fluid) p.flags[n] = (fl & uint8_t(~kFGI)) | kIG;
+    if ((m_if > 1.001f * rho) || not_gas) p.flags[n] = (fl & uint8_t(~kFGI)) | kIF;
+    else if ((m_if < -0.001f) || not_fluid) p.flags[n] = (fl & uint8_t(~kFGI)) | kIG;
   }
   float fi[9];
   bool forced = false;

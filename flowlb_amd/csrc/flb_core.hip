@@ -333,23 +333,27 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
   const int st = h->opt.storage;
   const bool fs = h->opt.problem == FLB_FREE_SURFACE;
   flb::Params p = params(h, bx);
-  auto run = [&](int k, flb::KernelFn fn) -> int {
+  auto run = [&](int k, flb::KernelFn fn, const flb::Params& q, int threads) -> int {
     if (ev) FLB_HIP(h, hipEventRecord(ev[2 * k], h->stream));
-    int rc = launch(h, fn, p, bx);
+    int rc = launch(h, fn, q, threads);
     if (rc) return rc;
     if (ev) FLB_HIP(h, hipEventRecord(ev[2 * k + 1], h->stream));
     return FLB_OK;
   };
   int rc;
   if (fs) {
-    if ((rc = run(FLB_K_FS1, flb::get_fs1(st, odd)))) return rc;
-    if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, true)))) return rc;
-    if ((rc = run(FLB_K_FS2, flb::get_fs2()))) return rc;
-    if ((rc = run(FLB_K_FS3, flb::get_fs3(st, odd)))) return rc;
-    if ((rc = run(FLB_K_FS4, flb::get_fs4()))) return rc;
-    if (h->opt.mass_mode == FLB_MASS_AS_WRITTEN) h->cur ^= 1;  // pass 4 wrote the other buffer
+    if ((rc = run(FLB_K_FS1, flb::get_fs1(st, odd), p, bx))) return rc;
+    if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, true), p, bx)))
+      return rc;
+    flb::Params q = p;  // passes 2-4: one block per kFsTileW x kFsTileH tile
+    q.tiles_x = (h->nx + flb::kFsTileW - 1) / flb::kFsTileW;
+    q.nblocks = q.tiles_x * ((h->ny + flb::kFsTileH - 1) / flb::kFsTileH);
+    if ((rc = run(FLB_K_FS_UPDATE, flb::get_fs_update(st, odd), q, flb::kFsTileW * flb::kFsTileH)))
+      return rc;
+    h->cur ^= 1;  // the update wrote the other flag buffer
   } else {
-    if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, false, moments)))) return rc;
+    if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, false, moments), p, bx)))
+      return rc;
   }
   h->t += 1;
   return FLB_OK;
@@ -415,7 +419,7 @@ int flb_step_timed(flb_handle* h, uint64_t nsteps, float* total_ms, float* per_k
   for (uint64_t s = 0; s < nsteps && !rc; ++s) {
     rc = enqueue_step(h, s + 1 == nsteps, ev);
     if (rc) break;
-    FLB_HIP(h, hipEventSynchronize(ev[2 * (fs ? FLB_K_FS4 : FLB_K_STREAM_COLLIDE) + 1]));
+    FLB_HIP(h, hipEventSynchronize(ev[2 * (fs ? FLB_K_FS_UPDATE : FLB_K_STREAM_COLLIDE) + 1]));
     for (int k = 0; k < FLB_NUM_KERNELS; ++k) {
       if (!fs && k != FLB_K_STREAM_COLLIDE) continue;
       float ms = 0.0f;

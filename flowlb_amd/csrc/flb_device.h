@@ -59,6 +59,10 @@ struct StoreBF16S {
   }
 };
 
+// Tile of the fused free-surface passes 2-4: one block of kFsTileW x kFsTileH
+// threads per tile, one wavefront per tile row.
+constexpr int kFsTileW = 64, kFsTileH = 8;
+
 // ---------------------------------------------------------------- params
 struct Params {
   uint32_t nx, ny;
@@ -70,7 +74,7 @@ struct Params {
   int32_t use_force, collision, smagorinsky, gas_check_corrected, mass_mode;
   void* f;               // populations, 9 planes of Np elements of the storage type
   uint8_t* flags;        // current flags
-  uint8_t* flags_next;   // flags written by the last free-surface pass
+  uint8_t* flags_next;   // flags written by the free-surface update (passes 2-4)
   float* rho;            // readback density (N)
   float* u;              // readback velocity (2N)
   float* pub_rho[2];     // densities published for inlet cells, by parity

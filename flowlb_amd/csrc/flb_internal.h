@@ -16,9 +16,7 @@ using ConvInFn = void (*)(const Params, const float*);
 using HistFn = void (*)(const Params, unsigned long long*);
 KernelFn get_stream_collide(int storage, int odd, bool fs, bool mom);
 KernelFn get_fs1(int storage, int odd);
-KernelFn get_fs2();
-KernelFn get_fs3(int storage, int odd);
-KernelFn get_fs4();
+KernelFn get_fs_update(int storage, int odd);
 KernelFn get_stream_collide_trace(int odd, bool mom);
 MassFn get_pop_mass(int storage);
 MassFn get_fs_mass();

@@ -16,7 +16,8 @@
 //     array that stream-collide merges when it loads the cell;
 //   * the gas->interface and interface->fluid neighbour rules are evaluated by
 //     the cell that changes (gather form), never written into a neighbour;
-//   * pass 4 reads the flags of the previous pass and writes a second buffer.
+//   * passes 2-4 run as one kernel on an LDS tile of the flags and write the
+//     new flags to a second buffer.
 #include "flb_cell.h"
 
 namespace flb {
@@ -38,27 +39,31 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
   const uint32_t n = c.n;
   const uint64_t N = p.N;
   const uint8_t fl = p.flags[n];
-  if (fl & (kGas | kWall | kInlet)) return;
   uint32_t nb[9];
   neighbours(p, c.x, c.y, nb);
+  uint8_t nf[9];
+  neighbour_flags(p, nb, nf);  // issued with the own flag: one round trip
+  if (fl & (kGas | kWall | kInlet)) return;
+  const bool cons = p.mass_mode != 0;
   // Conservative mode: the mass of a fluid cell whose eight neighbours are all
   // fluid or interface is updated by stream-collide instead (interior_fluid).
-  if (p.mass_mode != 0 && interior_fluid(p, fl, nb)) return;
-  float in[9], out[9];
+  if (cons && interior_fluid(fl, nf)) return;
+  float in[9], out[9], ex[9];
   load_incoming<S, ODD>(p, nb, in);
   out[0] = in[0];
   load_outgoing<S, ODD>(p, nb, out);
+#pragma unroll
+  for (int i = 1; i < 9; i++) ex[i] = p.excess[nb[i]];
   // As written the increments are added to the mass one by one in FP32
   // (lbmKernel2D.cu:38-52, 94-106).  The conservative mode sums them in FP64
   // and rounds once, so excess shares far below the rounding step of the
   // mass are not lost (FP32 one-by-one drifts ~1e-5 relative per 10^4 steps).
-  const bool cons = p.mass_mode != 0;
   float m = p.mass[n];
   double dm = 0.0;
 #pragma unroll
   for (int i = 1; i < 9; i++) {
-    if (cons) dm += double(p.excess[nb[i]]);
-    else m += p.excess[nb[i]];
+    if (cons) dm += double(ex[i]);
+    else m += ex[i];
   }
   const uint8_t e = fl & kFGI;
   if (e == kFluid) {
@@ -71,10 +76,23 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
     } else {
 #pragma unroll
       for (int i = 1; i < 9; i += 2) {
-        const uint8_t a = p.flags[nb[i]] & kFGI, b = p.flags[nb[i + 1]] & kFGI;
+        const uint8_t a = nf[i], b = nf[i + 1];
         if (a == kFluid || a == kInterface) dm += double(in[i + 1] - out[i]);
         if (b == kFluid || b == kInterface) dm += double(in[i] - out[i + 1]);
       }
+      // Deviation from the density stream-collide computes from the same
+      // incoming populations in the same order (an outlet keeps its density).
+      float rho;
+      if (fl & kOutlet) {
+        rho = p.rho[n];
+      } else {
+        float s = in[0];
+#pragma unroll
+        for (int i = 1; i < 9; i++) s += in[i];
+        rho = 1.0f + s;
+      }
+      p.mass[n] = float(dm + (double(m) - double(rho)));
+      return;
     }
   } else if (e == kInterface) {
     float rho, ux, uy;
@@ -90,21 +108,21 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
       ux = (out[1] - out[2] + out[5] - out[6] + out[7] - out[8]) / rho;
       uy = (out[3] - out[4] + out[5] - out[6] + out[8] - out[7]) / rho;
     }
+    float ph[9];
+#pragma unroll
+    for (int i = 1; i < 9; i++) ph[i] = p.phi[nb[i]];
     const float phi0 = cons ? p.phi[n] : fill_level(m, rho);
     const float rhol2 = 0.5f / rho;
     ux = ux + rhol2 * p.fx;
     uy = uy + rhol2 * p.fy;
     float fe[9];
     equilibrium_shifted(1.0f, 0.0f, ux, uy, fe);  // atmospheric equilibrium, rho = 1
-    uint8_t nf[9];
-#pragma unroll
-    for (int i = 1; i < 9; i++) nf[i] = p.flags[nb[i]] & kFGI;
 #pragma unroll
     for (int i = 1; i < 9; i += 2) {
       float d = 0.0f;
       bool any = true;
       if (nf[i] == kFluid) d = in[i + 1] - out[i];
-      else if (nf[i] == kInterface) d = (in[i + 1] - out[i]) * 0.5f * (p.phi[nb[i]] + phi0);
+      else if (nf[i] == kInterface) d = (in[i + 1] - out[i]) * 0.5f * (ph[i] + phi0);
       else any = false;
       if (any) {
         if (cons) dm += double(d);
@@ -112,7 +130,7 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
       }
       any = true;
       if (nf[i + 1] == kFluid) d = in[i] - out[i + 1];
-      else if (nf[i + 1] == kInterface) d = (in[i] - out[i + 1]) * 0.5f * (p.phi[nb[i + 1]] + phi0);
+      else if (nf[i + 1] == kInterface) d = (in[i] - out[i + 1]) * 0.5f * (ph[i + 1] + phi0);
       else any = false;
       if (any) {
         if (cons) dm += double(d);
@@ -130,70 +148,7 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
   p.mass[n] = m;
 }
 
-// d_FreeSurface2D_2 (src/cuda/lbmKernel2D.cu:123-148) in gather form: a gas or
-// interface->gas cell next to an interface->fluid cell changes itself.
-__global__ void __launch_bounds__(1024) k_fs2(const Params p) {
-  const Cell c = locate(p);
-  if (!c.valid) return;
-  const uint8_t fl = p.flags[c.n];
-  const uint8_t e = fl & kIG;
-  if (e != kIG && e != kGas) return;
-  uint32_t nb[9];
-  neighbours(p, c.x, c.y, nb);
-  bool near_if = false;
-#pragma unroll
-  for (int i = 1; i < 9; i++) near_if = near_if || ((p.flags[nb[i]] & kIF) == kIF);
-  if (!near_if) return;
-  p.flags[c.n] = e == kIG ? uint8_t(fl & ~kGas) : uint8_t((fl & ~kGas) | kGasInterface);
-}
-
-// d_FreeSurface2D_3 (src/cuda/lbmKernel2D.cu:150-190): gas->interface cells
-// take feq of the neighbour averages; fluid cells next to an interface->gas
-// cell become interface (gather form).
-template <class S, int ODD>
-__global__ void __launch_bounds__(1024) k_fs3(const Params p) {
-  const Cell c = locate(p);
-  if (!c.valid) return;
-  const uint32_t n = c.n;
-  const uint64_t N = p.N;
-  const uint8_t fl = p.flags[n];
-  const bool gi = (fl & (kGasInterface | kIG)) == kGasInterface;
-  const uint8_t fi = fl & (kInletFluid | kInterface);
-  const bool cand = fi == kFluid || fi == kIF;
-  if (!gi && !cand) return;
-  uint32_t nb[9];
-  neighbours(p, c.x, c.y, nb);
-  if (gi) {
-    // calculateAverageValuesNeighbors2D (cellCalculations.cuh:124-145)
-    float cnt = 0.0f, sr = 0.0f, sx = 0.0f, sy = 0.0f;
-#pragma unroll
-    for (int i = 1; i < 9; i++) {
-      if (p.flags[nb[i]] & kIF) {
-        cnt += 1.0f;
-        sr += p.rho[nb[i]];
-        sx += p.u[nb[i]];
-        sy += p.u[N + nb[i]];
-      }
-    }
-    float rho = cnt > 0.0f ? sr / cnt : 0.0f;
-    float ux = cnt > 0.0f ? sx / cnt : 0.0f;
-    float uy = cnt > 0.0f ? sy / cnt : 0.0f;
-    const float rhol2 = 0.5f / rho;
-    ux = ux + rhol2 * p.fx;
-    uy = uy + rhol2 * p.fy;
-    float fe[9];
-    equilibrium_shifted(rho, rho - 1.0f, ux, uy, fe);
-    store_outgoing<S, ODD>(p, nb, fe);
-  } else {
-    bool near_ig = false;
-#pragma unroll
-    for (int i = 1; i < 9; i++)
-      near_ig = near_ig || ((p.flags[nb[i]] & (kGasInterface | kIG)) == kIG);
-    if (near_ig) p.flags[n] = uint8_t((fl & ~kIF) | kInterface);
-  }
-}
-
-// Flag a cell carries after its own pass-4 transition.
+// Flag transition a cell applies to itself in pass 4.
 __device__ __forceinline__ uint8_t settle(uint8_t fl) {
   if (fl & (kWall | kMovingWall | kInlet)) return fl;
   const uint8_t e = fl & (kFGI | kGasInterface);
@@ -202,28 +157,105 @@ __device__ __forceinline__ uint8_t settle(uint8_t fl) {
   return fl;
 }
 
-// d_FreeSurface2D_4 (src/cuda/lbmKernel2D.cu:192-303): flag transitions,
-// mass clamping, excess mass split over the receiving neighbours.
-__global__ void __launch_bounds__(1024) k_fs4(const Params p) {
-  const Cell c = locate(p);
-  if (!c.valid) return;
-  const uint32_t n = c.n;
-  // As written the receivers are counted on the flags before this pass, so the
-  // new flags go to the second buffer.  In the conservative mode the count
-  // uses the settled flag of each neighbour, which is the same before and
-  // after its own transition: flags are updated in place.
-  const bool cons = p.mass_mode != 0;
-  const uint8_t fl = p.flags[n];
+// ---------------------------------------------------------- passes 2, 3, 4
+// d_FreeSurface2D_2, _3 and _4 (src/cuda/lbmKernel2D.cu:123-303) fused into one
+// launch.  The flags a cell sees in pass 4 depend on the flags within three
+// cells of it after stream-collide (pass 2 looks one cell further than pass 3,
+// pass 3 one further than pass 4), so each block loads its tile of flags with
+// a three-cell halo into LDS once and evaluates
+//   pass 2 on the tile + 2 rings:  gas / interface->gas cell next to an
+//          interface->fluid cell -> gas->interface / interface (gather form);
+//   pass 3 on the tile + 1 ring:   fluid / interface->fluid cell next to an
+//          interface->gas cell -> interface (gather form);
+//   pass 4 on the tile:            transitions, mass clamp, excess split;
+// plus the gas->interface initialisation of pass 3 (populations from the
+// neighbour averages) for its own cells.  Every cell is evaluated from the
+// same input flags by every block that holds it, so blocks agree without
+// communicating, and the result equals running the three passes as separate
+// kernels.  The new flags go to the second flag buffer.
+constexpr int kTileW = kFsTileW, kTileH = kFsTileH, kHalo = 3;
+constexpr int kLdsW = kTileW + 2 * kHalo, kLdsH = kTileH + 2 * kHalo;
+constexpr int kUpdateThreads = kTileW * kTileH;
+
+__device__ __forceinline__ int wrap(int v, int n) {
+  int w = v % n;
+  return w < 0 ? w + n : w;
+}
+
+// LDS offsets of the eight neighbours, in the order of neighbours().
+__device__ __forceinline__ int lds_offset(int i) {
+  switch (i) {
+    case 1: return 1;
+    case 2: return -1;
+    case 3: return kLdsW;
+    case 4: return -kLdsW;
+    case 5: return kLdsW + 1;
+    case 6: return -kLdsW - 1;
+    case 7: return -kLdsW + 1;
+    default: return kLdsW - 1;
+  }
+}
+
+template <class S, int ODD>
+__global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
+  __shared__ uint8_t f0s[kLdsH * kLdsW], f1s[kLdsH * kLdsW], f2s[kLdsH * kLdsW];
+  const int nx = int(p.nx), ny = int(p.ny);
+  const uint32_t b = xcd_tile(blockIdx.x, p.nblocks);
+  const uint32_t ty = b / p.tiles_x, tx = b - ty * p.tiles_x;
+  const int x0 = int(tx) * kTileW, y0 = int(ty) * kTileH;
+  const int tid = int(threadIdx.x);
+  const int x = x0 + (tid & (kTileW - 1)), y = y0 + tid / kTileW;
+  const bool valid = x < nx && y < ny;
+  const uint32_t n = valid ? uint32_t(x + y * nx) : 0u;
+  // The own mass does not depend on any flag: its load overlaps the tile's.
+  float m = valid ? p.mass[n] : 0.0f;
+  for (int i = tid; i < kLdsH * kLdsW; i += kUpdateThreads) {
+    const int r = i / kLdsW, col = i - r * kLdsW;
+    f0s[i] = p.flags[wrap(x0 + col - kHalo, nx) + wrap(y0 + r - kHalo, ny) * nx];
+  }
+  __syncthreads();
+  for (int i = tid; i < (kLdsH - 2) * (kLdsW - 2); i += kUpdateThreads) {  // pass 2
+    const int r = 1 + i / (kLdsW - 2), j = r * kLdsW + 1 + i % (kLdsW - 2);
+    const uint8_t fl = f0s[j];
+    const uint8_t e = fl & kIG;
+    uint8_t o = fl;
+    if (e == kIG || e == kGas) {
+      bool near_if = false;
+#pragma unroll
+      for (int k = 1; k < 9; k++) near_if = near_if || ((f0s[j + lds_offset(k)] & kIF) == kIF);
+      if (near_if) o = e == kIG ? uint8_t(fl & ~kGas) : uint8_t((fl & ~kGas) | kGasInterface);
+    }
+    f1s[j] = o;
+  }
+  __syncthreads();
+  for (int i = tid; i < (kLdsH - 4) * (kLdsW - 4); i += kUpdateThreads) {  // pass 3 flags
+    const int r = 2 + i / (kLdsW - 4), j = r * kLdsW + 2 + i % (kLdsW - 4);
+    const uint8_t fl = f1s[j];
+    const uint8_t fi = fl & (kInletFluid | kInterface);
+    uint8_t o = fl;
+    if (fi == kFluid || fi == kIF) {
+      bool near_ig = false;
+#pragma unroll
+      for (int k = 1; k < 9; k++)
+        near_ig = near_ig || ((f1s[j + lds_offset(k)] & (kGasInterface | kIG)) == kIG);
+      if (near_ig) o = uint8_t((fl & ~kIF) | kInterface);
+    }
+    f2s[j] = o;
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int j = (tid / kTileW + kHalo) * kLdsW + (tid & (kTileW - 1)) + kHalo;
+  const uint8_t f0 = f0s[j], fl = f2s[j];
   if (fl & (kWall | kMovingWall | kInlet)) {
-    if (!cons) p.flags_next[n] = fl;
+    p.flags_next[n] = fl;
     return;
   }
+  const bool cons = p.mass_mode != 0;
   const uint8_t e = fl & (kFGI | kGasInterface);
-  float m = p.mass[n];
   if (e == kGas && m == 0.0f) {
     // Empty gas cell.  A cell that turned gas in the previous step left an
     // excess share for its neighbours and marked its mass -0.0; clear both.
-    if (!cons) p.flags_next[n] = fl;
+    p.flags_next[n] = fl;
     if (signbit(m)) {
       p.mass[n] = 0.0f;
       p.excess[n] = 0.0f;
@@ -231,15 +263,46 @@ __global__ void __launch_bounds__(1024) k_fs4(const Params p) {
     return;
   }
   const float rho = p.rho[n];
+  if (e == kGasInterface) {
+    // Pass 3: calculateAverageValuesNeighbors2D (cellCalculations.cuh:124-145)
+    // over the fluid/interface neighbours, then feq with the Guo shift.
+    uint32_t nb[9];
+    neighbours(p, uint32_t(x), uint32_t(y), nb);
+    const uint64_t N = p.N;
+    float cnt = 0.0f, sr = 0.0f, sx = 0.0f, sy = 0.0f;
+#pragma unroll
+    for (int k = 1; k < 9; k++) {
+      if (f2s[j + lds_offset(k)] & kIF) {
+        cnt += 1.0f;
+        sr += p.rho[nb[k]];
+        sx += p.u[nb[k]];
+        sy += p.u[N + nb[k]];
+      }
+    }
+    const float ar = cnt > 0.0f ? sr / cnt : 0.0f;
+    float ux = cnt > 0.0f ? sx / cnt : 0.0f;
+    float uy = cnt > 0.0f ? sy / cnt : 0.0f;
+    const float rhol2 = 0.5f / ar;
+    ux = ux + rhol2 * p.fx;
+    uy = uy + rhol2 * p.fy;
+    float fe[9];
+    equilibrium_shifted(ar, ar - 1.0f, ux, uy, fe);
+    store_outgoing<S, ODD>(p, nb, fe);
+  }
+  // A fluid cell carries its mass as the deviation from rho in the
+  // conservative mode (interior_fluid); one that pass 3 made interface takes
+  // its absolute mass back.
+  const bool dev = cons && (f0 & (kFGI | kInlet)) == kFluid;
+  if (dev && e == kInterface) m = m + rho;
   float ex = 0.0f, ph = 0.0f;
   bool write_phi = true;
   uint8_t nfl = fl;
   switch (e) {
     case kFluid:
-      ex = m - rho;
+      ex = dev ? m : m - rho;
       m = rho;
       ph = 1.0f;
-      write_phi = false;  // a cell that is fluid here was fluid after the last pass
+      write_phi = false;  // a cell that is fluid here was fluid after the last step
       break;
     case kGas:
       ex = m;
@@ -271,12 +334,14 @@ __global__ void __launch_bounds__(1024) k_fs4(const Params p) {
     default:
       break;
   }
-  uint32_t nb[9];
-  neighbours(p, c.x, c.y, nb);
+  // Receivers of the excess.  As written: neighbours fluid, interface,
+  // interface->fluid or gas->interface before the transitions
+  // (lbmKernel2D.cu:285-292).  Conservative: the cells that pull it in the
+  // next pass 1, i.e. not gas, wall or inlet after their own transition.
   int cont = 0;
 #pragma unroll
-  for (int i = 1; i < 9; i++) {
-    const uint8_t q = p.flags[nb[i]];
+  for (int k = 1; k < 9; k++) {
+    const uint8_t q = f2s[j + lds_offset(k)];
     if (!cons) {
       const uint8_t r = q & (kIF | kGasInterface);
       cont += int(r == kFluid || r == kInterface || r == kIF || r == kGasInterface);
@@ -292,8 +357,7 @@ __global__ void __launch_bounds__(1024) k_fs4(const Params p) {
   p.mass[n] = m;
   p.excess[n] = ex;
   if (write_phi) p.phi[n] = ph;
-  if (!cons) p.flags_next[n] = nfl;
-  else if (nfl != fl) p.flags[n] = nfl;
+  p.flags_next[n] = nfl;
 }
 
 // ------------------------------------------------------------- utilities
@@ -380,7 +444,7 @@ KernelFn stream_collide_fn(int odd, bool fs, bool mom) {
 template <class S>
 KernelFn fs1_fn(int odd) { return odd ? k_fs1<S, 1> : k_fs1<S, 0>; }
 template <class S>
-KernelFn fs3_fn(int odd) { return odd ? k_fs3<S, 1> : k_fs3<S, 0>; }
+KernelFn fs_update_fn(int odd) { return odd ? k_fs_update<S, 1> : k_fs_update<S, 0>; }
 
 }  // namespace flb
 
@@ -400,21 +464,19 @@ KernelFn get_fs1(int storage, int odd) {
     default: return fs1_fn<StoreF32>(odd);
   }
 }
-KernelFn get_fs3(int storage, int odd) {
+KernelFn get_fs_update(int storage, int odd) {
   switch (storage) {
-    case 1: return fs3_fn<StoreF16S>(odd);
-    case 2: return fs3_fn<StoreBF16S>(odd);
-    default: return fs3_fn<StoreF32>(odd);
+    case 1: return fs_update_fn<StoreF16S>(odd);
+    case 2: return fs_update_fn<StoreBF16S>(odd);
+    default: return fs_update_fn<StoreF32>(odd);
   }
 }
-KernelFn get_fs2() { return k_fs2; }
 KernelFn get_stream_collide_trace(int odd, bool mom) {
   if (mom) return odd ? k_stream_collide<StoreF32, 1, false, true, true>
                       : k_stream_collide<StoreF32, 0, false, true, true>;
   return odd ? k_stream_collide<StoreF32, 1, false, false, true>
              : k_stream_collide<StoreF32, 0, false, false, true>;
 }
-KernelFn get_fs4() { return k_fs4; }
 
 using MassFn = void (*)(const Params, double*);
 MassFn get_pop_mass(int storage) {

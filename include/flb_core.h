@@ -144,9 +144,10 @@ uint64_t flb_time(const flb_handle* h);
 /* Enqueue nsteps steps between two HIP events on the handle's stream and
    return the elapsed device time in ms.  With per_kernel_ms non-NULL (array of
    FLB_NUM_KERNELS floats) each kernel launch is bracketed by its own events and
-   the summed device time per kernel is returned instead. */
-enum { FLB_K_STREAM_COLLIDE = 0, FLB_K_FS1 = 1, FLB_K_FS2 = 2, FLB_K_FS3 = 3, FLB_K_FS4 = 4,
-       FLB_NUM_KERNELS = 5 };
+   the summed device time per kernel is returned instead.  FLB_K_FS_UPDATE is
+   the one launch that runs free-surface passes 2, 3 and 4
+   (d_FreeSurface2D_2/_3/_4, src/cuda/lbmKernel2D.cu:123-303). */
+enum { FLB_K_STREAM_COLLIDE = 0, FLB_K_FS1 = 1, FLB_K_FS_UPDATE = 2, FLB_NUM_KERNELS = 3 };
 int flb_step_timed(flb_handle* h, uint64_t nsteps, float* total_ms, float* per_kernel_ms);
 
 /* Histogram of flags (256 counts), computed on the device. */
@@ -227,6 +228,28 @@ int flb_batch_info(const flb_batch* b, int32_t* block_x, int32_t* grid, int32_t*
                    uint32_t* ntiles);
 const char* flb_batch_last_error(const flb_batch* b);
 void flb_batch_destroy(flb_batch* b);
+
+/* ------------------------------------------------------------ output */
+/* VTK ImageData writer, VTIWriter (src/io/writer.cpp:15-252).  ASCII output
+   reproduces the reference's text layout; binary = 1 writes the same arrays
+   as raw appended data (UInt64 block headers), for large grids. */
+enum { FLB_VTI_UINT8 = 0, FLB_VTI_FLOAT32 = 1 };
+typedef struct flb_vti_field {
+  const char* name;
+  int32_t type;          /* FLB_VTI_UINT8 (1 component) or FLB_VTI_FLOAT32 */
+  uint32_t components;   /* 1, or 2 for [x | y] vector fields */
+  const void* data;      /* N (or 2N) values */
+  double scale;          /* multiplies every value (lattice -> SI) */
+  int32_t flip_y;        /* negate the y component */
+} flb_vti_field;
+int flb_write_vti(const char* path, uint32_t nx, uint32_t ny, double origin_x, double origin_y,
+                  double spacing, const flb_vti_field* fields, uint32_t nfields, int32_t binary);
+/* loaderFiles.vti.series (Writer::createTimeSeriesFile, src/io/writer.cpp:159-181). */
+int flb_write_vti_series(const char* path, uint32_t count, double time_interval);
+/* Readback + write of Flags, U, rho (and phi for free surface) of a handle. */
+int flb_save_vti(flb_handle* h, const char* path, uint32_t nx, uint32_t ny, int32_t free_surface,
+                 double origin_x, double origin_y, double spacing, double velocity_scale,
+                 double density_scale, int32_t binary);
 
 /* ------------------------------------------------------------ host set-up */
 
