@@ -12,7 +12,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libflbcore.so"
 
 FLUID, GAS, INTERFACE, WALL, MOVING_WALL, INLET, OUTLET, GAS_INTERFACE = (
     0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x40, 0x80)
-STORAGE = {"fp32": 0, "fp16s": 1, "bf16s": 2}
+STORAGE = {"fp32": 0, "fp16s": 1, "bf16s": 2, "fp64": 3}
 MONOFLUID, FREE_SURFACE = 0, 1     # flb_options.problem
 SRT, TRT = 0, 1                    # flb_options.collision
 K_STREAM_COLLIDE, K_FS1, K_FS_UPDATE = range(3)
@@ -123,6 +123,9 @@ PROTOTYPES = [
     ("flb_read", C.c_int, [_H, _F, _F, _U8, _F]),
     ("flb_read_fs", C.c_int, [_H, _F, _F]),
     ("flb_read_populations", C.c_int, [_H, _F]),
+    ("flb_read_f64", C.c_int, [_H, _D, _D, _U8, _D]),
+    ("flb_read_fs_f64", C.c_int, [_H, _D, _D]),
+    ("flb_read_populations_f64", C.c_int, [_H, _D]),
     ("flb_write_populations", C.c_int, [_H, _F]),
     ("flb_device_fields", C.c_int, [_H, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
@@ -154,8 +157,10 @@ PROTOTYPES = [
     ("flb_write_vti", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                 C.c_double, C.POINTER(VtiField), C.c_uint32, C.c_int32]),
     ("flb_write_vti_series", C.c_int, [C.c_char_p, C.c_uint32, C.c_double]),
-    ("flb_save_vti", C.c_int, [_H, C.c_char_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_double,
-                               C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]),
+    ("flb_save_vti", C.c_int, [_H, C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_int32]),
+    ("flb_get_shape", C.c_int, [_H, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                C.POINTER(C.c_int32)]),
     ("flb_build_channel_flags", C.c_int, [C.POINTER(Channel), _U8]),
     ("flb_seed_free_surface", C.c_int, [_U8, C.c_uint32, C.c_uint32]),
     ("flb_init_free_surface_fields", C.c_int, [_U8, C.c_size_t, _F, _F, _F]),
@@ -297,6 +302,28 @@ class Solver:
         check(lib().flb_read_populations(self.h, fptr(f)), self.h, "flb_read_populations")
         return f
 
+    def read64(self):
+        """read() in double precision (exact for FP64 storage)."""
+        N = self.N
+        d = lambda n: np.empty(n, np.float64)
+        u, rho, fl = d(2 * N), d(N), np.empty(N, np.uint8)
+        phi = d(N) if self.free_surface else None
+        P = lambda a: None if a is None else a.ctypes.data_as(_D)
+        check(lib().flb_read_f64(self.h, P(u), P(rho), u8ptr(fl), P(phi)), self.h, "flb_read_f64")
+        out = dict(u=u, rho=rho, flags=fl)
+        if phi is not None:
+            out["phi"] = phi
+            m, e = d(N), d(N)
+            check(lib().flb_read_fs_f64(self.h, P(m), P(e)), self.h, "flb_read_fs_f64")
+            out["mass"], out["excess"] = m, e
+        return out
+
+    def populations64(self):
+        f = np.empty(9 * self.N, np.float64)
+        check(lib().flb_read_populations_f64(self.h, f.ctypes.data_as(_D)), self.h,
+              "flb_read_populations_f64")
+        return f
+
     def write_populations(self, f):
         f = _f32(f, 9 * self.N)
         check(lib().flb_write_populations(self.h, fptr(f)), self.h, "flb_write_populations")
@@ -341,8 +368,7 @@ class Solver:
 
     def save_vti(self, path, spacing=1.0, origin=(0.0, 0.0), velocity_scale=1.0,
                  density_scale=1.0, binary=False):
-        check(lib().flb_save_vti(self.h, str(path).encode(), self.nx, self.ny,
-                                 int(self.free_surface), origin[0], origin[1], spacing,
+        check(lib().flb_save_vti(self.h, str(path).encode(), origin[0], origin[1], spacing,
                                  velocity_scale, density_scale, int(binary)), self.h, "flb_save_vti")
 
     def close(self):

@@ -9,39 +9,46 @@ namespace flb {
 
 // Equilibrium minus weights (feq_i - w_i), calculateEquilibriumDDF2Q9
 // (src/cuda/cellCalculations.cuh:32-52) rearranged for shifted storage:
-// feq_i - w_i = w_i * ((rho - 1) + rho * A_i).
-__device__ __forceinline__ void equilibrium_shifted(float rho, float rho1, float ux, float uy,
-                                                    float* fe) {
-  const float u2 = 3.0f * (ux * ux + uy * uy);
-  const float ax = 3.0f * ux, ay = 3.0f * uy;
-  const float a0 = ax - ay, a1 = ax + ay;
-  const float h = 0.5f * u2;
-  const float w0 = wgt(0), w1 = wgt(1), w5 = wgt(5);
+// feq_i - w_i = w_i * ((rho - 1) + rho * A_i).  R is the arithmetic type.
+template <class R>
+__device__ __forceinline__ void equilibrium_shifted(R rho, R rho1, R ux, R uy, R* fe) {
+  const R u2 = R(3) * (ux * ux + uy * uy);
+  const R ax = R(3) * ux, ay = R(3) * uy;
+  const R a0 = ax - ay, a1 = ax + ay;
+  const R h = R(0.5f) * u2;
+  const R w0 = wgt_r<R>(0), w1 = wgt_r<R>(1), w5 = wgt_r<R>(5);
+  const R q = R(0.5f);
   fe[0] = w0 * (rho1 - rho * h);
-  fe[1] = w1 * (rho1 + rho * (ax - h + 0.5f * ax * ax));
-  fe[2] = w1 * (rho1 + rho * (-ax - h + 0.5f * ax * ax));
-  fe[3] = w1 * (rho1 + rho * (ay - h + 0.5f * ay * ay));
-  fe[4] = w1 * (rho1 + rho * (-ay - h + 0.5f * ay * ay));
-  fe[5] = w5 * (rho1 + rho * (a1 - h + 0.5f * a1 * a1));
-  fe[6] = w5 * (rho1 + rho * (-a1 - h + 0.5f * a1 * a1));
-  fe[7] = w5 * (rho1 + rho * (a0 - h + 0.5f * a0 * a0));
-  fe[8] = w5 * (rho1 + rho * (-a0 - h + 0.5f * a0 * a0));
+  fe[1] = w1 * (rho1 + rho * (ax - h + q * ax * ax));
+  fe[2] = w1 * (rho1 + rho * (-ax - h + q * ax * ax));
+  fe[3] = w1 * (rho1 + rho * (ay - h + q * ay * ay));
+  fe[4] = w1 * (rho1 + rho * (-ay - h + q * ay * ay));
+  fe[5] = w5 * (rho1 + rho * (a1 - h + q * a1 * a1));
+  fe[6] = w5 * (rho1 + rho * (-a1 - h + q * a1 * a1));
+  fe[7] = w5 * (rho1 + rho * (a0 - h + q * a0 * a0));
+  fe[8] = w5 * (rho1 + rho * (-a0 - h + q * a0 * a0));
 }
 
 // Guo forcing terms, calculateForcingTerms2D (src/cuda/cellCalculations.cuh:64-73),
-// with the reference's literal 0.33333334f.
-__device__ __forceinline__ void guo_terms(float ux, float uy, float gx, float gy, float* fi) {
-  const float uF = -0.33333334f * (ux * gx + uy * gy);
-  fi[0] = 9.0f * wgt(0) * uF;
+// with the reference's literal 0.33333334f.  The force is a float constant
+// there, so c_i . F is formed in float also in the double path.
+template <class R>
+__device__ __forceinline__ void guo_terms(R ux, R uy, float gx, float gy, R* fi) {
+  const R uF = R(-0.33333334f) * (ux * R(gx) + uy * R(gy));
+  fi[0] = R(9.0f) * wgt_r<R>(0) * uF;
 #pragma unroll
   for (int i = 1; i < 9; i++) {
     const float cF = float(cx(i)) * gx + float(cy(i)) * gy;
-    fi[i] = 9.0f * wgt(i) * (cF * (float(cx(i)) * ux + float(cy(i)) * uy + 0.33333334f) + uF);
+    fi[i] = R(9.0f) * wgt_r<R>(i) *
+            (R(cF) * (R(cx(i)) * ux + R(cy(i)) * uy + R(0.33333334f)) + uF);
   }
 }
 
-__device__ __forceinline__ float fill_level(float m, float rho) {  // calculatePhi
-  return rho > 0.0f ? __saturatef(m / rho) : 0.5f;
+// calculatePhi (cellCalculations.cuh:153-158): __saturatef takes a float, so
+// the double path rounds the fill level to float as well.
+template <class R>
+__device__ __forceinline__ R fill_level(R m, R rho) {
+  return rho > R(0) ? R(__saturatef(float(m / rho))) : R(0.5f);
 }
 
 struct Cell {
@@ -110,7 +117,8 @@ __device__ __forceinline__ bool interior_fluid(uint8_t fl, const uint8_t* nf) {
 
 // loadf (src/cuda/cellCalculations.cuh:185-196) with ODD = t % 2.
 template <class S, int ODD>
-__device__ __forceinline__ void load_incoming(const Params& p, const uint32_t* nb, float* g) {
+__device__ __forceinline__ void load_incoming(const Params& p, const uint32_t* nb,
+                                              typename S::R* g) {
   const typename S::T* f = static_cast<const typename S::T*>(p.f);
   const uint64_t Np = p.Np;
   g[0] = S::load(f, nb[0]);
@@ -123,7 +131,8 @@ __device__ __forceinline__ void load_incoming(const Params& p, const uint32_t* n
 
 // loadOutgoingf (src/cuda/cellCalculations.cuh:207-217).
 template <class S, int ODD>
-__device__ __forceinline__ void load_outgoing(const Params& p, const uint32_t* nb, float* o) {
+__device__ __forceinline__ void load_outgoing(const Params& p, const uint32_t* nb,
+                                              typename S::R* o) {
   const typename S::T* f = static_cast<const typename S::T*>(p.f);
   const uint64_t Np = p.Np;
 #pragma unroll
@@ -135,7 +144,8 @@ __device__ __forceinline__ void load_outgoing(const Params& p, const uint32_t* n
 
 // storef (src/cuda/cellCalculations.cuh:228-238).
 template <class S, int ODD>
-__device__ __forceinline__ void store_outgoing(const Params& p, const uint32_t* nb, const float* g) {
+__device__ __forceinline__ void store_outgoing(const Params& p, const uint32_t* nb,
+                                               const typename S::R* g) {
   typename S::T* f = static_cast<typename S::T*>(p.f);
   const uint64_t Np = p.Np;
   S::store(f, nb[0], g[0]);
@@ -154,6 +164,13 @@ __device__ __forceinline__ void store_outgoing(const Params& p, const uint32_t* 
 // p.trace for the adjoint sweep.
 template <class S, int ODD, bool FS, bool MOM, bool TRACE>
 __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell c) {
+  using R = typename S::R;
+  R* const rho_f = fld<R>(p.rho);
+  R* const u_f = fld<R>(p.u);
+  R* const prho0 = fld<R>(p.pub_rho[ODD]);
+  R* const prho1 = fld<R>(p.pub_rho[ODD ^ 1]);
+  R* const pu0 = fld<R>(p.pub_u[ODD]);
+  R* const pu1 = fld<R>(p.pub_u[ODD ^ 1]);
   const uint32_t n = c.n;
   const uint64_t N = p.N;
   // Density of cell n is read next step by an inlet at n-1, velocity by an
@@ -167,34 +184,34 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   const bool pub_v = n + 1 < N && (fl_next & kOutlet) && !(fl & kInlet);
   if (fl & (kGas | kWall)) {
     if (FS) {  // a source cell that is gas this step keeps its last value
-      if (pub_r) p.pub_rho[ODD][n] = p.pub_rho[ODD ^ 1][n];
+      if (pub_r) prho0[n] = prho1[n];
       if (pub_v) {
-        p.pub_u[ODD][n] = p.pub_u[ODD ^ 1][n];
-        p.pub_u[ODD][N + n] = p.pub_u[ODD ^ 1][N + n];
+        pu0[n] = pu1[n];
+        pu0[N + n] = pu1[N + n];
       }
     }
     return;
   }
   uint32_t nb[9];
   neighbours(p, c.x, c.y, nb);
-  float g[9];
+  R g[9];
   load_incoming<S, ODD>(p, nb, g);
   // Free surface: every load that depends only on the own flag is issued here,
   // with the populations, so that the cell waits for memory once.
   uint8_t nf[9];
-  float ex[9], rc[9], m_if = 0.0f;
+  R ex[9], rc[9], m_if = R(0);
   const bool cons = FS && p.mass_mode != 0;
   const bool fluid_dev = cons && (fl & (kFGI | kInlet | kOutlet)) == kFluid;
   const bool iface = FS && (fl & kFGI) == kInterface;
   if (FS) neighbour_flags(p, nb, nf);
   if (fluid_dev) {
 #pragma unroll
-    for (int i = 1; i < 9; i++) ex[i] = p.excess[nb[i]];
+    for (int i = 1; i < 9; i++) ex[i] = fld<R>(p.excess)[nb[i]];
   }
   if (iface) {
-    m_if = p.mass[n];
+    m_if = fld<R>(p.mass)[n];
 #pragma unroll
-    for (int k = 1; k < 9; k++) rc[k] = p.rec[k * N + n];
+    for (int k = 1; k < 9; k++) rc[k] = fld<R>(p.rec)[k * N + n];
     // Reconstructed gas populations of free-surface pass 1 replace incoming
     // ones (storeReconstructedGasf, cellCalculations.cuh:250-259).
 #pragma unroll
@@ -205,22 +222,22 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     }
   }
   const uint8_t io = fl & (kInlet | kOutlet);
-  float rho, rho1, ux, uy;
+  R rho, rho1, ux, uy;
   if (io & kInlet) {
-    rho = p.pub_rho[ODD ^ 1][n + 1];
-    rho1 = rho - 1.0f;
-    ux = p.u[n];
-    uy = p.u[N + n];
+    rho = prho1[n + 1];
+    rho1 = rho - R(1);
+    ux = u_f[n];
+    uy = u_f[N + n];
     if (TRACE) {
       p.trace[n] = rho;
       p.trace[N + n] = ux;
       p.trace[2 * N + n] = uy;
     }
   } else if (fl & kOutlet) {
-    rho = p.rho[n];
-    rho1 = rho - 1.0f;
-    ux = p.pub_u[ODD ^ 1][n - 1];
-    uy = p.pub_u[ODD ^ 1][N + n - 1];
+    rho = rho_f[n];
+    rho1 = rho - R(1);
+    ux = pu1[n - 1];
+    uy = pu1[N + n - 1];
     if (TRACE) {
       p.trace[n] = rho;
       p.trace[N + n] = ux;
@@ -230,9 +247,9 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     rho1 = g[0];
 #pragma unroll
     for (int i = 1; i < 9; i++) rho1 += g[i];
-    rho = 1.0f + rho1;
-    const float mx = g[1] - g[2] + g[5] - g[6] + g[7] - g[8];
-    const float my = g[3] - g[4] + g[5] - g[6] + g[8] - g[7];
+    rho = R(1) + rho1;
+    const R mx = g[1] - g[2] + g[5] - g[6] + g[7] - g[8];
+    const R my = g[3] - g[4] + g[5] - g[6] + g[8] - g[7];
     ux = mx / rho;
     uy = my / rho;
     if (TRACE) {
@@ -245,7 +262,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     double m = 0.0;
 #pragma unroll
     for (int i = 1; i < 9; i++) m += double(ex[i]);
-    p.mass[n] = float(m);
+    fld<R>(p.mass)[n] = R(m);
   }
   if (iface) {  // interface -> fluid / gas test (:353-367)
     bool not_fluid = true, not_gas = true;
@@ -257,57 +274,59 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     if ((m_if > 1.001f * rho) || not_gas) p.flags[n] = (fl & uint8_t(~kFGI)) | kIF;
     else if ((m_if < -0.001f) || not_fluid) p.flags[n] = (fl & uint8_t(~kFGI)) | kIG;
   }
-  float fi[9];
+  R fi[9];
   bool forced = false;
   if (p.alpha && !io) {
     // Brinkman drag G = F - alpha rho u with the Guo half-force shift solved
-    // for u (no reference counterpart).
-    const float a = p.alpha[n];
-    const float gx0 = p.use_force ? p.fx : 0.0f, gy0 = p.use_force ? p.fy : 0.0f;
-    const float rhol2 = 0.5f / rho;
-    const float s = 1.0f / (1.0f + 0.5f * a);
+    // for u (no reference counterpart; FP32 storage only).
+    const R a = R(p.alpha[n]);
+    const R gx0 = R(p.use_force ? p.fx : 0.0f), gy0 = R(p.use_force ? p.fy : 0.0f);
+    const R rhol2 = R(0.5f) / rho;
+    const R s = R(1) / (R(1) + R(0.5f) * a);
     ux = (ux + rhol2 * gx0) * s;
     uy = (uy + rhol2 * gy0) * s;
-    guo_terms(ux, uy, gx0 - a * rho * ux, gy0 - a * rho * uy, fi);
+    guo_terms<R>(ux, uy, float(gx0 - a * rho * ux), float(gy0 - a * rho * uy), fi);
     forced = true;
   } else if (p.use_force) {
-    const float rhol2 = 0.5f / rho;
-    ux = ux + rhol2 * p.fx;
-    uy = uy + rhol2 * p.fy;
-    guo_terms(ux, uy, p.fx, p.fy, fi);
+    const R rhol2 = R(0.5f) / rho;
+    ux = ux + rhol2 * R(p.fx);
+    uy = uy + rhol2 * R(p.fy);
+    guo_terms<R>(ux, uy, p.fx, p.fy, fi);
     forced = true;
   }
   if (MOM || FS) {
     if (!io) {
-      p.rho[n] = rho;
-      p.u[n] = ux;
-      p.u[N + n] = uy;
+      rho_f[n] = rho;
+      u_f[n] = ux;
+      u_f[N + n] = uy;
     } else if (fl & kOutlet) {
-      p.u[n] = ux;
-      p.u[N + n] = uy;
+      u_f[n] = ux;
+      u_f[N + n] = uy;
     }
-    p.rho[n] = rho;
+    rho_f[n] = rho;
   }
-  if (pub_r) p.pub_rho[ODD][n] = rho;
+  if (pub_r) prho0[n] = rho;
   if (pub_v) {
-    p.pub_u[ODD][n] = ux;
-    p.pub_u[ODD][N + n] = uy;
+    pu0[n] = ux;
+    pu0[N + n] = uy;
   }
-  float fe[9];
-  equilibrium_shifted(rho, rho1, ux, uy, fe);
+  R fe[9];
+  equilibrium_shifted<R>(rho, rho1, ux, uy, fe);
+  // 1/tau is a float constant in the reference also for PRECISION = 64, and
+  // the Smagorinsky correction goes through float sqrtf (:414-442).
   float inv_tau = p.inv_tau;
-  if (p.smagorinsky) {  // (:417-442)
-    float hxx = 0.0f, hyy = 0.0f, hxy = 0.0f;
+  if (p.smagorinsky) {
+    R hxx = R(0), hyy = R(0), hxy = R(0);
 #pragma unroll
     for (int i = 1; i < 9; i++) {
-      const float d = g[i] - fe[i];
-      hxx += float(cx(i) * cx(i)) * d;
-      hyy += float(cy(i) * cy(i)) * d;
-      hxy += float(cx(i) * cy(i)) * d;
+      const R d = g[i] - fe[i];
+      hxx += R(cx(i) * cx(i)) * d;
+      hyy += R(cy(i) * cy(i)) * d;
+      hxy += R(cx(i) * cy(i)) * d;
     }
-    const float Q = hxx * hxx + hyy * hyy + 2.0f * hxy * hxy;
+    const R Q = hxx * hxx + hyy * hyy + R(2.0f) * hxy * hxy;
     const float tau = 1.0f / inv_tau;
-    inv_tau = 2.0f / (tau + sqrtf(tau * tau + 0.54037965f * sqrtf(Q) / rho));
+    inv_tau = 2.0f / (tau + sqrtf(float(R(tau * tau) + R(0.54037965f * sqrtf(float(Q))) / rho)));
   }
   if (io) {
 #pragma unroll
@@ -317,22 +336,23 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     if (forced) {
       const float tl = 1.0f - 0.5f * inv_tau;
 #pragma unroll
-      for (int i = 0; i < 9; i++) g[i] = omc * g[i] + inv_tau * fe[i] + tl * fi[i];
+      for (int i = 0; i < 9; i++) g[i] = R(omc) * g[i] + R(inv_tau) * fe[i] + R(tl) * fi[i];
     } else {
 #pragma unroll
-      for (int i = 0; i < 9; i++) g[i] = omc * g[i] + inv_tau * fe[i];
+      for (int i = 0; i < 9; i++) g[i] = R(omc) * g[i] + R(inv_tau) * fe[i];
     }
   } else if (!forced) {  // TRT, magic parameter 1/4; the forced variant only streams
+    // The reference keeps these temporaries in float also for PRECISION = 64.
     const float ip = inv_tau;
     const float in = 1.0f / (0.25f / (1.0f / inv_tau - 0.5f) + 0.5f);
-    g[0] = g[0] - ip * (g[0] - fe[0]);
+    g[0] = g[0] - R(ip) * (g[0] - fe[0]);
 #pragma unroll
     for (int i = 1; i < 9; i += 2) {
-      const float sym = g[i] - fe[i] + g[i + 1] - fe[i + 1];
-      const float fa = g[i] - g[i + 1];
-      const float fea = fe[i] - fe[i + 1];
-      g[i] = g[i] - 0.5f * ip * sym - 0.5f * in * (fa - fea);
-      g[i + 1] = g[i + 1] - 0.5f * ip * sym - 0.5f * in * (fea - fa);
+      const float sym = float(g[i] - fe[i] + g[i + 1] - fe[i + 1]);
+      const float fa = float(g[i] - g[i + 1]);
+      const float fea = float(fe[i] - fe[i + 1]);
+      g[i] = g[i] - R(0.5f * ip * sym) - R(0.5f * in * (fa - fea));
+      g[i + 1] = g[i + 1] - R(0.5f * ip * sym) - R(0.5f * in * (fea - fa));
     }
   }
   store_outgoing<S, ODD>(p, nb, g);

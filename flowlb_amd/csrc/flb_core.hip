@@ -13,6 +13,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/flb_core.h"
@@ -104,6 +105,47 @@ int download(flb_handle* h, void* dst, const void* src, size_t bytes) {
   return FLB_OK;
 }
 
+// Cell fields hold float, or double for FP64 storage (h->field_bytes); the
+// host side of the ABI passes float arrays and these helpers convert.
+int field_alloc(flb_handle* h, float** ptr, size_t count) {
+  uint8_t* b = nullptr;
+  int rc = dev_alloc(h, &b, count * size_t(h->field_bytes));
+  *ptr = reinterpret_cast<float*>(b);
+  return rc;
+}
+
+template <class In>
+int upload_field(flb_handle* h, float* dst, const In* src, size_t count) {
+  if (h->field_bytes == 8) {
+    std::vector<double> t(src, src + count);
+    return upload(h, dst, t.data(), count * sizeof(double));
+  }
+  if (std::is_same<In, float>::value) return upload(h, dst, src, count * sizeof(float));
+  std::vector<float> t(src, src + count);
+  return upload(h, dst, t.data(), count * sizeof(float));
+}
+
+template <class Out>
+int download_field(flb_handle* h, Out* dst, const float* src, size_t count) {
+  if (h->field_bytes == 8) {
+    if (std::is_same<Out, double>::value) return download(h, dst, src, count * sizeof(double));
+    std::vector<double> t(count);
+    int rc = download(h, t.data(), src, count * sizeof(double));
+    for (size_t i = 0; i < count && !rc; ++i) dst[i] = Out(t[i]);
+    return rc;
+  }
+  if (std::is_same<Out, float>::value) return download(h, dst, src, count * sizeof(float));
+  std::vector<float> t(count);
+  int rc = download(h, t.data(), src, count * sizeof(float));
+  for (size_t i = 0; i < count && !rc; ++i) dst[i] = Out(t[i]);
+  return rc;
+}
+
+int zero_field(flb_handle* h, float* dst, size_t count) {
+  FLB_HIP(h, hipMemsetAsync(dst, 0, count * size_t(h->field_bytes), h->stream));
+  return FLB_OK;
+}
+
 int launch(flb_handle* h, flb::KernelFn fn, const flb::Params& p, int bx) {
   hipLaunchKernelGGL(fn, dim3(p.nblocks), dim3(bx), 0, h->stream, p);
   FLB_HIP(h, hipGetLastError());
@@ -113,10 +155,11 @@ int launch(flb_handle* h, flb::KernelFn fn, const flb::Params& p, int bx) {
 // Host initial populations, initFields (src/initData.h:125-165): feq(rho0, u0)
 // evaluated in double as initEquilibriumDDF2Q9 does (:75-100), stored with
 // the odd-step pattern (storef with t = 1), DDF-shifted (f - w).
+template <class T>
 void initial_populations(uint32_t nx, uint32_t ny, const float* rho0, const float* u0,
-                         std::vector<float>& fs) {
+                         std::vector<T>& fs) {
   const size_t N = size_t(nx) * ny;
-  fs.assign(9 * N, 0.0f);
+  fs.assign(9 * N, T(0));
   const double w[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
                        1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
   const int cxs[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
@@ -145,17 +188,17 @@ void initial_populations(uint32_t nx, uint32_t ny, const float* rho0, const floa
                             size_t(yd) * nx + xr,
                             size_t(yu) * nx + xl};
       // storef at odd parity: f_i -> slot (i+1, n + c_i), f_{i+1} -> slot (i, n).
-      fs[n] = float(feq[0]);
+      fs[n] = T(feq[0]);
       for (int i = 1; i < 9; i += 2) {
-        fs[size_t(i + 1) * N + nb[i]] = float(feq[i]);
-        fs[size_t(i) * N + n] = float(feq[i + 1]);
+        fs[size_t(i + 1) * N + nb[i]] = T(feq[i]);
+        fs[size_t(i) * N + n] = T(feq[i + 1]);
       }
     }
 }
 
 int validate(const flb_options* o, uint32_t nx, uint32_t ny, std::string& why) {
   if (!(o->tau > 0.5f) || !std::isfinite(o->tau)) return why = "tau must be > 0.5", 1;
-  if (o->storage < 0 || o->storage > 2) return why = "unknown storage format", 1;
+  if (o->storage < 0 || o->storage > 3) return why = "unknown storage format", 1;
   if (o->collision < 0 || o->collision > 1) return why = "collision must be SRT or TRT", 1;
   if (o->problem < 0 || o->problem > 1) return why = "problem must be monofluid or free surface", 1;
   if (o->mass_mode < 0 || o->mass_mode > 1) return why = "unknown mass mode", 1;
@@ -196,6 +239,14 @@ const char* flb_last_error(const flb_handle* h) { return h ? h->err.c_str() : ""
 
 uint64_t flb_time(const flb_handle* h) { return h ? h->t : 0; }
 
+int flb_get_shape(const flb_handle* h, uint32_t* nx, uint32_t* ny, int32_t* problem) {
+  if (!h) return FLB_ERR_ARGUMENT;
+  if (nx) *nx = h->nx;
+  if (ny) *ny = h->ny;
+  if (problem) *problem = h->opt.problem;
+  return FLB_OK;
+}
+
 void flb_destroy(flb_handle* h) {
   if (!h) return;
   if (h->stream) {
@@ -226,7 +277,8 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   h->ny = ny;
   h->N = N;
   h->Np = round_up(N, 64);
-  h->elem_bytes = opt->storage == FLB_STORAGE_FP32 ? 4 : 2;
+  h->elem_bytes = opt->storage == FLB_STORAGE_FP32 ? 4 : (opt->storage == FLB_STORAGE_FP64 ? 8 : 2);
+  h->field_bytes = opt->storage == FLB_STORAGE_FP64 ? 8 : 4;
   h->inv_tau = 1.0f / opt->tau;
   auto bail = [&](int rc) {
     flb_destroy(h);
@@ -249,17 +301,17 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   const bool fs = opt->problem == FLB_FREE_SURFACE;
   if ((rc = dev_alloc(h, &h->flags[0], N))) return bail(rc);
   if ((rc = dev_alloc(h, &h->flags[1], fs ? N : 1))) return bail(rc);
-  if ((rc = dev_alloc(h, &h->rho, N))) return bail(rc);
-  if ((rc = dev_alloc(h, &h->u, 2 * N))) return bail(rc);
+  if ((rc = field_alloc(h, &h->rho, N))) return bail(rc);
+  if ((rc = field_alloc(h, &h->u, 2 * N))) return bail(rc);
   for (int k = 0; k < 2; ++k) {
-    if ((rc = dev_alloc(h, &h->pub_rho[k], N))) return bail(rc);
-    if ((rc = dev_alloc(h, &h->pub_u[k], 2 * N))) return bail(rc);
+    if ((rc = field_alloc(h, &h->pub_rho[k], N))) return bail(rc);
+    if ((rc = field_alloc(h, &h->pub_u[k], 2 * N))) return bail(rc);
   }
   if (fs) {
-    if ((rc = dev_alloc(h, &h->mass, N))) return bail(rc);
-    if ((rc = dev_alloc(h, &h->excess, N))) return bail(rc);
-    if ((rc = dev_alloc(h, &h->phi, N))) return bail(rc);
-    if ((rc = dev_alloc(h, &h->rec, 9 * N))) return bail(rc);
+    if ((rc = field_alloc(h, &h->mass, N))) return bail(rc);
+    if ((rc = field_alloc(h, &h->excess, N))) return bail(rc);
+    if ((rc = field_alloc(h, &h->phi, N))) return bail(rc);
+    if ((rc = field_alloc(h, &h->rec, 9 * N))) return bail(rc);
   }
   if ((rc = dev_alloc(h, &h->rho_init, N))) return bail(rc);
   if ((rc = dev_alloc(h, &h->u_init, 2 * N))) return bail(rc);
@@ -274,34 +326,42 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   if (u0) std::memcpy(hu.data(), u0, 2 * N * sizeof(float));
   if ((rc = upload(h, h->flags[0], flags, N))) return bail(rc);
   if (fs && (rc = upload(h, h->flags[1], flags, N))) return bail(rc);
-  if ((rc = upload(h, h->rho, hr.data(), N * sizeof(float)))) return bail(rc);
-  if ((rc = upload(h, h->u, hu.data(), 2 * N * sizeof(float)))) return bail(rc);
+  if ((rc = upload_field(h, h->rho, hr.data(), N))) return bail(rc);
+  if ((rc = upload_field(h, h->u, hu.data(), 2 * N))) return bail(rc);
   if ((rc = upload(h, h->rho_init, hr.data(), N * sizeof(float)))) return bail(rc);
   if ((rc = upload(h, h->u_init, hu.data(), 2 * N * sizeof(float)))) return bail(rc);
   for (int k = 0; k < 2; ++k) {
-    if ((rc = upload(h, h->pub_rho[k], hr.data(), N * sizeof(float)))) return bail(rc);
-    if ((rc = upload(h, h->pub_u[k], hu.data(), 2 * N * sizeof(float)))) return bail(rc);
+    if ((rc = upload_field(h, h->pub_rho[k], hr.data(), N))) return bail(rc);
+    if ((rc = upload_field(h, h->pub_u[k], hu.data(), 2 * N))) return bail(rc);
   }
   if (fs) {
-    FLB_HIP(h, hipMemsetAsync(h->mass, 0, N * sizeof(float), h->stream));
-    FLB_HIP(h, hipMemsetAsync(h->excess, 0, N * sizeof(float), h->stream));
-    FLB_HIP(h, hipMemsetAsync(h->phi, 0, N * sizeof(float), h->stream));
-    FLB_HIP(h, hipMemsetAsync(h->rec, 0, 9 * N * sizeof(float), h->stream));
+    if ((rc = zero_field(h, h->mass, N)) || (rc = zero_field(h, h->excess, N)) ||
+        (rc = zero_field(h, h->phi, N)) || (rc = zero_field(h, h->rec, 9 * N)))
+      return bail(rc);
   }
-  // Populations: shifted feq, converted to the storage format on the device.
+  // Populations: shifted feq, converted to the storage format on the device
+  // (from double for FP64 storage, from float otherwise).
   {
-    std::vector<float> fsh;
-    initial_populations(nx, ny, rho0, u0, fsh);
-    float* staging = nullptr;
-    if (hipMalloc(&staging, fsh.size() * sizeof(float)) != hipSuccess) {
+    const bool f64 = opt->storage == FLB_STORAGE_FP64;
+    std::vector<double> fsd;
+    std::vector<float> fsf;
+    if (f64) initial_populations(nx, ny, rho0, u0, fsd);
+    else initial_populations(nx, ny, rho0, u0, fsf);
+    const size_t bytes = f64 ? fsd.size() * sizeof(double) : fsf.size() * sizeof(float);
+    void* staging = nullptr;
+    if (hipMalloc(&staging, bytes) != hipSuccess) {
       (void)hipGetLastError();
       return bail(FLB_ERR_MEMORY);
     }
-    int r2 = upload(h, staging, fsh.data(), fsh.size() * sizeof(float));
+    int r2 = upload(h, staging, f64 ? static_cast<const void*>(fsd.data()) : fsf.data(), bytes);
     if (!r2) {
       const flb::Params p = params(h, kUtilBlock);
-      hipLaunchKernelGGL(flb::get_f32_to_pop(opt->storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
-                         h->stream, p, staging);
+      if (f64)
+        hipLaunchKernelGGL(flb::get_f64_to_pop(opt->storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
+                           h->stream, p, static_cast<const double*>(staging));
+      else
+        hipLaunchKernelGGL(flb::get_f32_to_pop(opt->storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
+                           h->stream, p, static_cast<const float*>(staging));
       hipError_t e = hipGetLastError();
       if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
       if (e != hipSuccess) r2 = FLB_ERR_DEVICE;
@@ -316,13 +376,12 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
 int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const float* excess) {
   if (!h || !mass || !phi) return h ? fail(h, FLB_ERR_ARGUMENT, "mass and phi required") : FLB_ERR_ARGUMENT;
   if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "not a free-surface handle");
-  const size_t b = h->N * sizeof(float);
+  const size_t n = h->N;
   int rc;
-  if ((rc = upload(h, h->mass, mass, b))) return rc;
-  if ((rc = upload(h, h->phi, phi, b))) return rc;
-  if (excess) return upload(h, h->excess, excess, b);
-  FLB_HIP(h, hipMemsetAsync(h->excess, 0, b, h->stream));
-  return FLB_OK;
+  if ((rc = upload_field(h, h->mass, mass, n))) return rc;
+  if ((rc = upload_field(h, h->phi, phi, n))) return rc;
+  if (excess) return upload_field(h, h->excess, excess, n);
+  return zero_field(h, h->excess, n);
 }
 
 // One lattice step on the handle's stream.  ev (optional) receives per-kernel
@@ -364,7 +423,7 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
 namespace flb {
 void host_initial_populations(uint32_t nx, uint32_t ny, const float* rho0, const float* u0,
                               std::vector<float>& fs) {
-  initial_populations(nx, ny, rho0, u0, fs);
+  initial_populations<float>(nx, ny, rho0, u0, fs);
 }
 int set_error(flb_handle* h, int code, const std::string& msg) { return fail(h, code, msg); }
 Params make_params(const flb_handle* h, int bx) { return params(h, bx); }
@@ -433,41 +492,68 @@ int flb_step_timed(flb_handle* h, uint64_t nsteps, float* total_ms, float* per_k
   return rc;
 }
 
-int flb_read(flb_handle* h, float* u, float* rho, uint8_t* flags, float* phi) {
+}  // extern "C"
+
+namespace {
+template <class T>
+int read_fields(flb_handle* h, T* u, T* rho, uint8_t* flags, T* phi) {
   if (!h) return FLB_ERR_ARGUMENT;
   const size_t N = h->N;
   int rc;
-  if (u && (rc = download(h, u, h->u, 2 * N * sizeof(float)))) return rc;
-  if (rho && (rc = download(h, rho, h->rho, N * sizeof(float)))) return rc;
+  if (u && (rc = download_field(h, u, h->u, 2 * N))) return rc;
+  if (rho && (rc = download_field(h, rho, h->rho, N))) return rc;
   if (flags && (rc = download(h, flags, h->flags[h->cur], N))) return rc;
   if (phi) {
     if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "phi needs free surface");
-    if ((rc = download(h, phi, h->phi, N * sizeof(float)))) return rc;
+    if ((rc = download_field(h, phi, h->phi, N))) return rc;
   }
   return FLB_OK;
 }
 
-int flb_read_fs(flb_handle* h, float* mass, float* excess) {
+template <class T>
+int read_fs_fields(flb_handle* h, T* mass, T* excess) {
   if (!h) return FLB_ERR_ARGUMENT;
   if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "not a free-surface handle");
   int rc;
-  if (mass && (rc = download(h, mass, h->mass, h->N * sizeof(float)))) return rc;
-  if (excess && (rc = download(h, excess, h->excess, h->N * sizeof(float)))) return rc;
+  if (mass && (rc = download_field(h, mass, h->mass, h->N))) return rc;
+  if (excess && (rc = download_field(h, excess, h->excess, h->N))) return rc;
   return FLB_OK;
 }
 
-int flb_read_populations(flb_handle* h, float* f) {
+template <class T>
+int read_pops(flb_handle* h, T* f) {
   if (!h || !f) return FLB_ERR_ARGUMENT;
-  float* tmp = nullptr;
-  FLB_HIP(h, hipMalloc(&tmp, 9 * h->N * sizeof(float)));
+  T* tmp = nullptr;
+  FLB_HIP(h, hipMalloc(&tmp, 9 * h->N * sizeof(T)));
   const flb::Params p = params(h, kUtilBlock);
-  hipLaunchKernelGGL(flb::get_pop_to_f32(h->opt.storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
-                     h->stream, p, tmp);
+  if (std::is_same<T, double>::value)
+    hipLaunchKernelGGL(flb::get_pop_to_f64(h->opt.storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
+                       h->stream, p, reinterpret_cast<double*>(tmp));
+  else
+    hipLaunchKernelGGL(flb::get_pop_to_f32(h->opt.storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
+                       h->stream, p, reinterpret_cast<float*>(tmp));
   hipError_t e = hipGetLastError();
-  int rc = e == hipSuccess ? download(h, f, tmp, 9 * h->N * sizeof(float)) : hip_fail(h, e, "pop_to_f32");
+  int rc = e == hipSuccess ? download(h, f, tmp, 9 * h->N * sizeof(T)) : hip_fail(h, e, "pop_out");
   (void)hipFree(tmp);
   return rc;
 }
+
+}  // namespace
+
+extern "C" {
+
+int flb_read(flb_handle* h, float* u, float* rho, uint8_t* flags, float* phi) {
+  return read_fields(h, u, rho, flags, phi);
+}
+int flb_read_f64(flb_handle* h, double* u, double* rho, uint8_t* flags, double* phi) {
+  return read_fields(h, u, rho, flags, phi);
+}
+int flb_read_fs(flb_handle* h, float* mass, float* excess) { return read_fs_fields(h, mass, excess); }
+int flb_read_fs_f64(flb_handle* h, double* mass, double* excess) {
+  return read_fs_fields(h, mass, excess);
+}
+int flb_read_populations(flb_handle* h, float* f) { return read_pops(h, f); }
+int flb_read_populations_f64(flb_handle* h, double* f) { return read_pops(h, f); }
 
 int flb_write_populations(flb_handle* h, const float* f) {
   if (!h || !f) return FLB_ERR_ARGUMENT;
@@ -505,7 +591,7 @@ int flb_mass(flb_handle* h, double* total) {
   if (!h || !total) return FLB_ERR_ARGUMENT;
   const flb::Params p = params(h, kUtilBlock);
   const bool fs = h->opt.problem == FLB_FREE_SURFACE;
-  hipLaunchKernelGGL(fs ? flb::get_fs_mass() : flb::get_pop_mass(h->opt.storage), dim3(p.nblocks),
+  hipLaunchKernelGGL(fs ? flb::get_fs_mass(h->opt.storage) : flb::get_pop_mass(h->opt.storage), dim3(p.nblocks),
                      dim3(kUtilBlock), 0, h->stream, p, h->partial);
   FLB_HIP(h, hipGetLastError());
   std::vector<double> part(h->partial_len);
@@ -525,7 +611,8 @@ int flb_flag_histogram(flb_handle* h, uint64_t* counts) {
   hipError_t e = hipMemsetAsync(d, 0, 256 * sizeof(unsigned long long), h->stream);
   if (e == hipSuccess) {
     const flb::Params p = params(h, kUtilBlock);
-    hipLaunchKernelGGL(flb::get_flag_hist(), dim3(p.nblocks), dim3(kUtilBlock), 0, h->stream, p, d);
+    const uint32_t blocks = p.nblocks < 2048u ? p.nblocks : 2048u;  // grid-stride kernel
+    hipLaunchKernelGGL(flb::get_flag_hist(), dim3(blocks), dim3(256), 0, h->stream, p, d);
     e = hipGetLastError();
   }
   if (e != hipSuccess) rc = hip_fail(h, e, "flag histogram");

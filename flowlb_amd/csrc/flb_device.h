@@ -27,19 +27,39 @@ enum : uint8_t {
   kFGI = 0x07, kIF = 0x05, kIG = 0x06, kInletFluid = 0x21
 };
 
+// Weights in the arithmetic type R: float as above, double for the FP64 path
+// (the host builds them in double there, src/ui/app.cpp:303).
+template <class R>
+__host__ __device__ constexpr R wgt_r(int i) {
+  return i == 0 ? R(4) / R(9) : (i < 5 ? R(1) / R(9) : R(1) / R(36));
+}
+
 // ---------------------------------------------------------------- storage
-// All formats hold the DDF-shifted population f_i - w_i; arithmetic is FP32.
+// All formats hold the DDF-shifted population f_i - w_i.  T is the stored
+// type, R the arithmetic type: FP32 for the 32- and 16-bit formats, FP64 for
+// StoreF64 (the reference's PRECISION = 64, whose fields -- density,
+// velocity, mass, excess, fill level -- are double as well).
 struct StoreF32 {
   using T = float;
+  using R = float;
   static constexpr int kBytes = 4;
   __device__ static float load(const T* p, size_t i) { return p[i]; }
   __device__ static void store(T* p, size_t i, float v) { p[i] = v; }
+};
+
+struct StoreF64 {
+  using T = double;
+  using R = double;
+  static constexpr int kBytes = 8;
+  __device__ static double load(const T* p, size_t i) { return p[i]; }
+  __device__ static void store(T* p, size_t i, double v) { p[i] = v; }
 };
 
 // IEEE binary16, scaled by 2^15 so that the shifted populations (|f - w| < 1)
 // sit in the normal range.
 struct StoreF16S {
   using T = _Float16;
+  using R = float;
   static constexpr int kBytes = 2;
   __device__ static float load(const T* p, size_t i) { return float(p[i]) * 3.0517578125e-05f; }
   __device__ static void store(T* p, size_t i, float v) { p[i] = _Float16(v * 32768.0f); }
@@ -48,6 +68,7 @@ struct StoreF16S {
 // bfloat16 (upper half of an FP32), round to nearest even on store.
 struct StoreBF16S {
   using T = uint16_t;
+  using R = float;
   static constexpr int kBytes = 2;
   __device__ static float load(const T* p, size_t i) {
     return __uint_as_float(uint32_t(p[i]) << 16);
@@ -75,6 +96,8 @@ struct Params {
   void* f;               // populations, 9 planes of Np elements of the storage type
   uint8_t* flags;        // current flags
   uint8_t* flags_next;   // flags written by the free-surface update (passes 2-4)
+  // Cell fields.  They hold elements of the storage's arithmetic type R
+  // (double for StoreF64); kernels access them through fld<R>().
   float* rho;            // readback density (N)
   float* u;              // readback velocity (2N)
   float* pub_rho[2];     // densities published for inlet cells, by parity
@@ -86,6 +109,12 @@ struct Params {
   const float* alpha;    // Brinkman solid fraction or nullptr
   float* trace;          // adjoint trace of one step: 3 planes of N (or nullptr)
 };
+
+// A cell field viewed in the arithmetic type R.
+template <class R>
+__device__ __forceinline__ R* fld(float* p) {
+  return reinterpret_cast<R*>(p);
+}
 
 // Block -> tile mapping.  Workgroups are dispatched round-robin over the 8
 // XCDs; remapping gives each XCD a contiguous range of tiles (rows), so the

@@ -13,15 +13,19 @@ using KernelFn = void (*)(const Params);
 using MassFn = void (*)(const Params, double*);
 using ConvOutFn = void (*)(const Params, float*);
 using ConvInFn = void (*)(const Params, const float*);
+using ConvOut64Fn = void (*)(const Params, double*);
+using ConvIn64Fn = void (*)(const Params, const double*);
 using HistFn = void (*)(const Params, unsigned long long*);
 KernelFn get_stream_collide(int storage, int odd, bool fs, bool mom);
 KernelFn get_fs1(int storage, int odd);
 KernelFn get_fs_update(int storage, int odd);
 KernelFn get_stream_collide_trace(int odd, bool mom);
 MassFn get_pop_mass(int storage);
-MassFn get_fs_mass();
+MassFn get_fs_mass(int storage);
 ConvOutFn get_pop_to_f32(int storage);
+ConvOut64Fn get_pop_to_f64(int storage);
 ConvInFn get_f32_to_pop(int storage);
+ConvIn64Fn get_f64_to_pop(int storage);
 HistFn get_flag_hist();
 }  // namespace flb
 
@@ -29,7 +33,8 @@ struct flb_handle {
   flb_options opt{};
   uint32_t nx = 0, ny = 0;
   uint64_t N = 0, Np = 0;
-  int elem_bytes = 4;
+  int elem_bytes = 4;   // bytes per stored population
+  int field_bytes = 4;  // bytes per element of the cell fields (8 for FP64)
   hipStream_t stream = nullptr;
   void* f = nullptr;
   uint8_t* flags[2] = {nullptr, nullptr};

@@ -135,11 +135,15 @@ extern "C" int flb_write_vti_series(const char* path, uint32_t count, double tim
 // Readback of a handle's fields into a .vti: Flags, U (scaled), rho (scaled)
 // and phi for free-surface handles, as the run loops save them
 // (src/cuda/runCuda.cu:483-487, 518-523).
-extern "C" int flb_save_vti(flb_handle* h, const char* path, uint32_t nx, uint32_t ny,
-                            int32_t free_surface, double origin_x, double origin_y,
+extern "C" int flb_save_vti(flb_handle* h, const char* path, double origin_x, double origin_y,
                             double spacing, double velocity_scale, double density_scale,
                             int32_t binary) {
   if (!h || !path) return FLB_ERR_ARGUMENT;
+  uint32_t nx = 0, ny = 0;
+  int32_t problem = 0;
+  int rc0 = flb_get_shape(h, &nx, &ny, &problem);
+  if (rc0) return rc0;
+  const bool free_surface = problem == FLB_FREE_SURFACE;
   const size_t N = size_t(nx) * ny;
   std::vector<float> u(2 * N), rho(N), phi(free_surface ? N : 0);
   std::vector<uint8_t> flags(N);

@@ -36,6 +36,9 @@ template <class S, int ODD>
 __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
   const Cell c = locate(p);
   if (!c.valid) return;
+  using R = typename S::R;
+  R* const mass = fld<R>(p.mass);
+  R* const phi = fld<R>(p.phi);
   const uint32_t n = c.n;
   const uint64_t N = p.N;
   const uint8_t fl = p.flags[n];
@@ -48,17 +51,17 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
   // Conservative mode: the mass of a fluid cell whose eight neighbours are all
   // fluid or interface is updated by stream-collide instead (interior_fluid).
   if (cons && interior_fluid(fl, nf)) return;
-  float in[9], out[9], ex[9];
+  R in[9], out[9], ex[9];
   load_incoming<S, ODD>(p, nb, in);
   out[0] = in[0];
   load_outgoing<S, ODD>(p, nb, out);
 #pragma unroll
-  for (int i = 1; i < 9; i++) ex[i] = p.excess[nb[i]];
+  for (int i = 1; i < 9; i++) ex[i] = fld<R>(p.excess)[nb[i]];
   // As written the increments are added to the mass one by one in FP32
   // (lbmKernel2D.cu:38-52, 94-106).  The conservative mode sums them in FP64
   // and rounds once, so excess shares far below the rounding step of the
   // mass are not lost (FP32 one-by-one drifts ~1e-5 relative per 10^4 steps).
-  float m = p.mass[n];
+  R m = mass[n];
   double dm = 0.0;
 #pragma unroll
   for (int i = 1; i < 9; i++) {
@@ -82,47 +85,47 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
       }
       // Deviation from the density stream-collide computes from the same
       // incoming populations in the same order (an outlet keeps its density).
-      float rho;
+      R rho;
       if (fl & kOutlet) {
-        rho = p.rho[n];
+        rho = fld<R>(p.rho)[n];
       } else {
-        float s = in[0];
+        R s = in[0];
 #pragma unroll
         for (int i = 1; i < 9; i++) s += in[i];
-        rho = 1.0f + s;
+        rho = R(1) + s;
       }
-      p.mass[n] = float(dm + (double(m) - double(rho)));
+      mass[n] = R(dm + (double(m) - double(rho)));
       return;
     }
   } else if (e == kInterface) {
-    float rho, ux, uy;
+    R rho, ux, uy;
     if (fl & (kInlet | kOutlet)) {
-      rho = p.rho[n];
-      ux = p.u[n];
-      uy = p.u[N + n];
+      rho = fld<R>(p.rho)[n];
+      ux = fld<R>(p.u)[n];
+      uy = fld<R>(p.u)[N + n];
     } else {
-      float s = out[0];
+      R s = out[0];
 #pragma unroll
       for (int i = 1; i < 9; i++) s += out[i];
-      rho = 1.0f + s;
+      rho = R(1) + s;
       ux = (out[1] - out[2] + out[5] - out[6] + out[7] - out[8]) / rho;
       uy = (out[3] - out[4] + out[5] - out[6] + out[8] - out[7]) / rho;
     }
-    float ph[9];
+    R ph[9];
 #pragma unroll
-    for (int i = 1; i < 9; i++) ph[i] = p.phi[nb[i]];
-    const float phi0 = cons ? p.phi[n] : fill_level(m, rho);
-    const float rhol2 = 0.5f / rho;
-    ux = ux + rhol2 * p.fx;
-    uy = uy + rhol2 * p.fy;
-    float fe[9];
-    equilibrium_shifted(1.0f, 0.0f, ux, uy, fe);  // atmospheric equilibrium, rho = 1
+    for (int i = 1; i < 9; i++) ph[i] = phi[nb[i]];
+    const R phi0 = cons ? phi[n] : fill_level<R>(m, rho);
+    const R rhol2 = R(0.5f) / rho;
+    ux = ux + rhol2 * R(p.fx);
+    uy = uy + rhol2 * R(p.fy);
+    R fe[9];
+    equilibrium_shifted<R>(R(1), R(0), ux, uy, fe);  // atmospheric equilibrium, rho = 1
 #pragma unroll
     for (int i = 1; i < 9; i += 2) {
-      float d = 0.0f;
+      R d = R(0);
       bool any = true;
       if (nf[i] == kFluid) d = in[i + 1] - out[i];
-      else if (nf[i] == kInterface) d = (in[i + 1] - out[i]) * 0.5f * (ph[i] + phi0);
+      else if (nf[i] == kInterface) d = (in[i + 1] - out[i]) * R(0.5f) * (ph[i] + phi0);
       else any = false;
       if (any) {
         if (cons) dm += double(d);
@@ -130,7 +133,7 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
       }
       any = true;
       if (nf[i + 1] == kFluid) d = in[i] - out[i + 1];
-      else if (nf[i + 1] == kInterface) d = (in[i] - out[i + 1]) * 0.5f * (ph[i + 1] + phi0);
+      else if (nf[i + 1] == kInterface) d = (in[i] - out[i + 1]) * R(0.5f) * (ph[i + 1] + phi0);
       else any = false;
       if (any) {
         if (cons) dm += double(d);
@@ -141,11 +144,11 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
     for (int k = 1; k < 9; k++) {
       const int opp = (k & 1) ? k + 1 : k - 1;
       const uint8_t gate = p.gas_check_corrected ? nf[opp] : nf[k];
-      if (gate == kGas) p.rec[k * N + n] = fe[k] - out[opp] + fe[opp];
+      if (gate == kGas) fld<R>(p.rec)[k * N + n] = fe[k] - out[opp] + fe[opp];
     }
   }
-  if (cons) m = float(double(m) + dm);
-  p.mass[n] = m;
+  if (cons) m = R(double(m) + dm);
+  mass[n] = m;
 }
 
 // Flag transition a cell applies to itself in pass 4.
@@ -198,6 +201,8 @@ __device__ __forceinline__ int lds_offset(int i) {
 
 template <class S, int ODD>
 __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
+  using R = typename S::R;
+  R* const mass = fld<R>(p.mass);
   __shared__ uint8_t f0s[kLdsH * kLdsW], f1s[kLdsH * kLdsW], f2s[kLdsH * kLdsW];
   const int nx = int(p.nx), ny = int(p.ny);
   const uint32_t b = xcd_tile(blockIdx.x, p.nblocks);
@@ -208,7 +213,7 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   const bool valid = x < nx && y < ny;
   const uint32_t n = valid ? uint32_t(x + y * nx) : 0u;
   // The own mass does not depend on any flag: its load overlaps the tile's.
-  float m = valid ? p.mass[n] : 0.0f;
+  R m = valid ? mass[n] : R(0);
   for (int i = tid; i < kLdsH * kLdsW; i += kUpdateThreads) {
     const int r = i / kLdsW, col = i - r * kLdsW;
     f0s[i] = p.flags[wrap(x0 + col - kHalo, nx) + wrap(y0 + r - kHalo, ny) * nx];
@@ -257,15 +262,16 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
     // excess share for its neighbours and marked its mass -0.0; clear both.
     p.flags_next[n] = fl;
     if (signbit(m)) {
-      p.mass[n] = 0.0f;
-      p.excess[n] = 0.0f;
+      mass[n] = R(0);
+      fld<R>(p.excess)[n] = R(0);
     }
     return;
   }
-  const float rho = p.rho[n];
+  const R rho = fld<R>(p.rho)[n];
   if (e == kGasInterface) {
     // Pass 3: calculateAverageValuesNeighbors2D (cellCalculations.cuh:124-145)
-    // over the fluid/interface neighbours, then feq with the Guo shift.
+    // over the fluid/interface neighbours, accumulated in float as the
+    // reference does for both precisions, then feq with the Guo shift.
     uint32_t nb[9];
     neighbours(p, uint32_t(x), uint32_t(y), nb);
     const uint64_t N = p.N;
@@ -274,19 +280,19 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
     for (int k = 1; k < 9; k++) {
       if (f2s[j + lds_offset(k)] & kIF) {
         cnt += 1.0f;
-        sr += p.rho[nb[k]];
-        sx += p.u[nb[k]];
-        sy += p.u[N + nb[k]];
+        sr += float(fld<R>(p.rho)[nb[k]]);
+        sx += float(fld<R>(p.u)[nb[k]]);
+        sy += float(fld<R>(p.u)[N + nb[k]]);
       }
     }
-    const float ar = cnt > 0.0f ? sr / cnt : 0.0f;
-    float ux = cnt > 0.0f ? sx / cnt : 0.0f;
-    float uy = cnt > 0.0f ? sy / cnt : 0.0f;
-    const float rhol2 = 0.5f / ar;
-    ux = ux + rhol2 * p.fx;
-    uy = uy + rhol2 * p.fy;
-    float fe[9];
-    equilibrium_shifted(ar, ar - 1.0f, ux, uy, fe);
+    const R ar = R(cnt > 0.0f ? sr / cnt : 0.0f);
+    R ux = R(cnt > 0.0f ? sx / cnt : 0.0f);
+    R uy = R(cnt > 0.0f ? sy / cnt : 0.0f);
+    const R rhol2 = R(0.5f) / ar;
+    ux = ux + rhol2 * R(p.fx);
+    uy = uy + rhol2 * R(p.fy);
+    R fe[9];
+    equilibrium_shifted<R>(ar, ar - R(1), ux, uy, fe);
     store_outgoing<S, ODD>(p, nb, fe);
   }
   // A fluid cell carries its mass as the deviation from rho in the
@@ -294,30 +300,30 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   // its absolute mass back.
   const bool dev = cons && (f0 & (kFGI | kInlet)) == kFluid;
   if (dev && e == kInterface) m = m + rho;
-  float ex = 0.0f, ph = 0.0f;
+  R ex = R(0), ph = R(0);
   bool write_phi = true;
   uint8_t nfl = fl;
   switch (e) {
     case kFluid:
       ex = dev ? m : m - rho;
       m = rho;
-      ph = 1.0f;
+      ph = R(1);
       write_phi = false;  // a cell that is fluid here was fluid after the last step
       break;
     case kGas:
       ex = m;
-      m = -0.0f;
+      m = -R(0);
       break;
     case kIF:
       nfl = fl & uint8_t(~kInterface);
       ex = m - rho;
       m = rho;
-      ph = 1.0f;
+      ph = R(1);
       break;
     case kIG:
       nfl = fl & uint8_t(~kInterface);
       ex = m;
-      m = -0.0f;
+      m = -R(0);
       break;
     case kInterface:
     case kGasInterface:
@@ -325,11 +331,11 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
       if (m > rho) {
         ex = m - rho;
         m = rho;
-      } else if (m < 0.0f) {
+      } else if (m < R(0)) {
         ex = m;
-        m = 0.0f;
+        m = R(0);
       }
-      ph = fill_level(m, rho);
+      ph = fill_level<R>(m, rho);
       break;
     default:
       break;
@@ -349,14 +355,14 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
       cont += int((settle(q) & (kGas | kWall | kInlet)) == 0);
     }
   }
-  if (cont > 0) ex = ex / float(cont);
+  if (cont > 0) ex = ex / R(float(cont));
   else {
     m += ex;
-    ex = 0.0f;
+    ex = R(0);
   }
-  p.mass[n] = m;
-  p.excess[n] = ex;
-  if (write_phi) p.phi[n] = ph;
+  mass[n] = m;
+  fld<R>(p.excess)[n] = ex;
+  if (write_phi) fld<R>(p.phi)[n] = ph;
   p.flags_next[n] = nfl;
 }
 
@@ -374,16 +380,17 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 }
 
 // Free-surface mass: stored mass + excess shares pulled next step, FP64.
+template <class R>
 __global__ void __launch_bounds__(256) k_fs_mass(const Params p, double* partial) {
   __shared__ double lds[16];
   const Cell c = locate(p);
   double v = 0.0;
   if (c.valid) {
-    v = p.mass[c.n];
+    v = double(fld<R>(p.mass)[c.n]);
     if (!(p.flags[c.n] & (kGas | kWall | kInlet))) {
       uint32_t nb[9];
       neighbours(p, c.x, c.y, nb);
-      for (int i = 1; i < 9; i++) v += double(p.excess[nb[i]]);
+      for (int i = 1; i < 9; i++) v += double(fld<R>(p.excess)[nb[i]]);
     }
   }
   const double s = block_sum(v, lds);
@@ -404,29 +411,47 @@ __global__ void __launch_bounds__(256) k_pop_mass(const Params p, double* partia
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-template <class S>
-__global__ void k_pop_to_f32(const Params p, float* out) {
+// Full populations f = (f - w) + w, in the output type O (float or double).
+template <class S, class O>
+__global__ void k_pop_out(const Params p, O* out) {
+  using R = typename S::R;
   const Cell c = locate(p);
   if (!c.valid) return;
   const typename S::T* f = static_cast<const typename S::T*>(p.f);
-  for (int q = 0; q < 9; q++) out[q * p.N + c.n] = S::load(f + q * p.Np, c.n) + wgt(q);
+  for (int q = 0; q < 9; q++) out[q * p.N + c.n] = O(S::load(f + q * p.Np, c.n) + wgt_r<R>(q));
 }
 
-// in: shifted populations (f - w), FP32, plane stride N.
-template <class S>
-__global__ void k_f32_to_pop(const Params p, const float* in) {
+// in: shifted populations (f - w) of type I, plane stride N.
+template <class S, class I>
+__global__ void k_pop_in(const Params p, const I* in) {
+  using R = typename S::R;
   const Cell c = locate(p);
   if (!c.valid) return;
   typename S::T* f = static_cast<typename S::T*>(p.f);
-  for (int q = 0; q < 9; q++) S::store(f + q * p.Np, c.n, in[q * p.N + c.n]);
+  for (int q = 0; q < 9; q++) S::store(f + q * p.Np, c.n, R(in[q * p.N + c.n]));
 }
 
-__global__ void k_flag_hist(const Params p, unsigned long long* counts) {
+// Flag histogram, grid-stride over all cells.  The lanes of a wavefront that
+// hold the same flag add their count once (one LDS atomic per distinct value
+// per wavefront), and each block adds its non-zero bins to the global counts
+// once: a per-cell atomic on a few hot bins (gas, fluid) serialises.
+__global__ void __launch_bounds__(256) k_flag_hist(const Params p, unsigned long long* counts) {
   __shared__ unsigned int h[256];
   for (int i = threadIdx.x; i < 256; i += blockDim.x) h[i] = 0;
   __syncthreads();
-  const Cell c = locate(p);
-  if (c.valid) atomicAdd(&h[p.flags[c.n]], 1u);
+  const int lane = threadIdx.x & 63;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; n < p.N; n += stride) {
+    const int v = p.flags[n];
+    uint64_t todo = __ballot(1);
+    while (todo) {
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const int lv = __shfl(v, leader, 64);
+      const uint64_t same = __ballot(v == lv);
+      if (lane == leader) atomicAdd(&h[lv], unsigned(__popcll(same)));
+      todo &= ~same;
+    }
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < 256; i += blockDim.x)
     if (h[i]) atomicAdd(&counts[i], (unsigned long long)h[i]);
@@ -434,6 +459,12 @@ __global__ void k_flag_hist(const Params p, unsigned long long* counts) {
 
 // ------------------------------------------------------------- launch table
 using KernelFn = void (*)(const Params);
+using MassFn = void (*)(const Params, double*);
+using ConvOutFn = void (*)(const Params, float*);
+using ConvInFn = void (*)(const Params, const float*);
+using ConvOut64Fn = void (*)(const Params, double*);
+using ConvIn64Fn = void (*)(const Params, const double*);
+using HistFn = void (*)(const Params, unsigned long long*);
 
 template <class S>
 KernelFn stream_collide_fn(int odd, bool fs, bool mom) {
@@ -449,11 +480,22 @@ KernelFn fs_update_fn(int odd) { return odd ? k_fs_update<S, 1> : k_fs_update<S,
 }  // namespace flb
 
 // Host-side accessors used by flb_core.hip (same translation unit group).
+// Storage codes: 0 FP32, 1 FP16S, 2 BF16S, 3 FP64.
 namespace flb {
+template <template <class> class Pick>
+auto by_storage(int storage) {
+  switch (storage) {
+    case 1: return Pick<StoreF16S>::get();
+    case 2: return Pick<StoreBF16S>::get();
+    case 3: return Pick<StoreF64>::get();
+    default: return Pick<StoreF32>::get();
+  }
+}
 KernelFn get_stream_collide(int storage, int odd, bool fs, bool mom) {
   switch (storage) {
     case 1: return stream_collide_fn<StoreF16S>(odd, fs, mom);
     case 2: return stream_collide_fn<StoreBF16S>(odd, fs, mom);
+    case 3: return stream_collide_fn<StoreF64>(odd, fs, mom);
     default: return stream_collide_fn<StoreF32>(odd, fs, mom);
   }
 }
@@ -461,6 +503,7 @@ KernelFn get_fs1(int storage, int odd) {
   switch (storage) {
     case 1: return fs1_fn<StoreF16S>(odd);
     case 2: return fs1_fn<StoreBF16S>(odd);
+    case 3: return fs1_fn<StoreF64>(odd);
     default: return fs1_fn<StoreF32>(odd);
   }
 }
@@ -468,6 +511,7 @@ KernelFn get_fs_update(int storage, int odd) {
   switch (storage) {
     case 1: return fs_update_fn<StoreF16S>(odd);
     case 2: return fs_update_fn<StoreBF16S>(odd);
+    case 3: return fs_update_fn<StoreF64>(odd);
     default: return fs_update_fn<StoreF32>(odd);
   }
 }
@@ -478,31 +522,35 @@ KernelFn get_stream_collide_trace(int odd, bool mom) {
              : k_stream_collide<StoreF32, 0, false, false, true>;
 }
 
-using MassFn = void (*)(const Params, double*);
-MassFn get_pop_mass(int storage) {
-  switch (storage) {
-    case 1: return k_pop_mass<StoreF16S>;
-    case 2: return k_pop_mass<StoreBF16S>;
-    default: return k_pop_mass<StoreF32>;
-  }
-}
-MassFn get_fs_mass() { return k_fs_mass; }
-using ConvOutFn = void (*)(const Params, float*);
-using ConvInFn = void (*)(const Params, const float*);
-ConvOutFn get_pop_to_f32(int storage) {
-  switch (storage) {
-    case 1: return k_pop_to_f32<StoreF16S>;
-    case 2: return k_pop_to_f32<StoreBF16S>;
-    default: return k_pop_to_f32<StoreF32>;
-  }
-}
-ConvInFn get_f32_to_pop(int storage) {
-  switch (storage) {
-    case 1: return k_f32_to_pop<StoreF16S>;
-    case 2: return k_f32_to_pop<StoreBF16S>;
-    default: return k_f32_to_pop<StoreF32>;
-  }
-}
-using HistFn = void (*)(const Params, unsigned long long*);
+template <class S>
+struct PopMass {
+  static MassFn get() { return k_pop_mass<S>; }
+};
+template <class S>
+struct FsMass {
+  static MassFn get() { return k_fs_mass<typename S::R>; }
+};
+template <class S>
+struct PopOut32 {
+  static ConvOutFn get() { return k_pop_out<S, float>; }
+};
+template <class S>
+struct PopOut64 {
+  static ConvOut64Fn get() { return k_pop_out<S, double>; }
+};
+template <class S>
+struct PopIn32 {
+  static ConvInFn get() { return k_pop_in<S, float>; }
+};
+template <class S>
+struct PopIn64 {
+  static ConvIn64Fn get() { return k_pop_in<S, double>; }
+};
+MassFn get_pop_mass(int storage) { return by_storage<PopMass>(storage); }
+MassFn get_fs_mass(int storage) { return by_storage<FsMass>(storage); }
+ConvOutFn get_pop_to_f32(int storage) { return by_storage<PopOut32>(storage); }
+ConvOut64Fn get_pop_to_f64(int storage) { return by_storage<PopOut64>(storage); }
+ConvInFn get_f32_to_pop(int storage) { return by_storage<PopIn32>(storage); }
+ConvIn64Fn get_f64_to_pop(int storage) { return by_storage<PopIn64>(storage); }
 HistFn get_flag_hist() { return k_flag_hist; }
 }  // namespace flb

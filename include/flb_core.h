@@ -49,9 +49,15 @@ enum {
   FLB_MOVING_WALL = 0x10, FLB_INLET = 0x20, FLB_OUTLET = 0x40, FLB_GAS_INTERFACE = 0x80
 };
 
-/* Population storage in device memory.  All arithmetic is FP32; the 16-bit
-   formats store f_i - w_i (DDF-shifted), FP16 scaled by 2^15. */
-enum { FLB_STORAGE_FP32 = 0, FLB_STORAGE_FP16S = 1, FLB_STORAGE_BF16S = 2 };
+/* Population storage in device memory.  All formats store f_i - w_i
+   (DDF-shifted; FP16 scaled by 2^15).  FP32, FP16S and BF16S compute in FP32
+   with FP32 cell fields.  FP64 is the reference's PRECISION = 64
+   (h_launchCudaCalculations2D<double>, src/ui/app.cpp:300-310): populations,
+   fields and arithmetic in FP64, with the float-typed constants and float
+   temporaries of the reference kept (1/tau, F, Smagorinsky sqrtf, TRT
+   temporaries, neighbour averages).  Free-surface and monofluid runs; no
+   adjoint or batch. */
+enum { FLB_STORAGE_FP32 = 0, FLB_STORAGE_FP16S = 1, FLB_STORAGE_BF16S = 2, FLB_STORAGE_FP64 = 3 };
 /* OptionsCalculation::typeProblem (src/io/reader.h:104). */
 enum { FLB_MONOFLUID = 0, FLB_FREE_SURFACE = 1 };
 /* OptionsCalculation::collisionOperator: 0 = SRT, 1 = TRT (src/cuda/lbmKernel2D.cu:444-511). */
@@ -118,18 +124,25 @@ int flb_sync(flb_handle* h);
    the fields the run loops save (src/cuda/runCuda.cu:483-487, 518-523):
    u (2N), rho (N), flags (N), phi (N, free surface only). */
 int flb_read(flb_handle* h, float* u, float* rho, uint8_t* flags, float* phi);
+/* The same in double: exact for FP64 storage, widened from float otherwise. */
+int flb_read_f64(flb_handle* h, double* u, double* rho, uint8_t* flags, double* phi);
 
-/* Free-surface mass and excess-mass fields (N floats each, NULL allowed). */
+/* Free-surface mass and excess-mass fields (N values each, NULL allowed). */
 int flb_read_fs(flb_handle* h, float* mass, float* excess);
+int flb_read_fs_f64(flb_handle* h, double* mass, double* excess);
 
 /* Raw population array f[q*N + n] in the esoteric-pull slot layout, as
-   unshifted FP32 values (9N floats). */
+   unshifted values (9N floats / doubles). */
 int flb_read_populations(flb_handle* h, float* f);
+int flb_read_populations_f64(flb_handle* h, double* f);
 int flb_write_populations(flb_handle* h, const float* f);
 
 /* Device pointers of the fields, for a HIP-GL interop consumer
    (the reference maps u onto a GL buffer, src/cuda/runCuda.cu:137-147).
-   Any output pointer may be NULL.  They stay valid until flb_destroy. */
+   Any output pointer may be NULL.  u, rho and phi hold doubles for FP64
+   storage, floats otherwise.  They stay valid until flb_destroy; the flag
+   buffer of a free-surface handle alternates every step, so query it again
+   after stepping. */
 int flb_device_fields(flb_handle* h, void** d_u, void** d_rho, void** d_flags, void** d_phi);
 
 /* Total fluid mass in FP64: stored mass plus the excess-mass shares that the
@@ -140,6 +153,10 @@ int flb_mass(flb_handle* h, double* total);
 
 /* Lattice time t (number of steps done). */
 uint64_t flb_time(const flb_handle* h);
+
+/* Grid size and problem type (FLB_MONOFLUID / FLB_FREE_SURFACE) of a handle;
+   any output pointer may be NULL. */
+int flb_get_shape(const flb_handle* h, uint32_t* nx, uint32_t* ny, int32_t* problem);
 
 /* Enqueue nsteps steps between two HIP events on the handle's stream and
    return the elapsed device time in ms.  With per_kernel_ms non-NULL (array of
@@ -246,10 +263,10 @@ int flb_write_vti(const char* path, uint32_t nx, uint32_t ny, double origin_x, d
                   double spacing, const flb_vti_field* fields, uint32_t nfields, int32_t binary);
 /* loaderFiles.vti.series (Writer::createTimeSeriesFile, src/io/writer.cpp:159-181). */
 int flb_write_vti_series(const char* path, uint32_t count, double time_interval);
-/* Readback + write of Flags, U, rho (and phi for free surface) of a handle. */
-int flb_save_vti(flb_handle* h, const char* path, uint32_t nx, uint32_t ny, int32_t free_surface,
-                 double origin_x, double origin_y, double spacing, double velocity_scale,
-                 double density_scale, int32_t binary);
+/* Readback + write of Flags, U, rho (and phi for free surface) of a handle
+   (the save step of h_runCuda*NoGraphics2D, src/cuda/runCuda.cu:483-487). */
+int flb_save_vti(flb_handle* h, const char* path, double origin_x, double origin_y,
+                 double spacing, double velocity_scale, double density_scale, int32_t binary);
 
 /* ------------------------------------------------------------ host set-up */
 

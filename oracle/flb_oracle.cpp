@@ -45,9 +45,9 @@ template <> struct Weights<double> {
                                   1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 };
 
-// __saturatef: clamp to [0, 1], NaN -> 0.
+// __saturatef: clamp to [0, 1], NaN -> 0.  It takes a float, so a double
+// argument is rounded to float first (the PRECISION = 64 kernels pass doubles).
 inline float saturate(float v) { return v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f; }
-inline double saturate(double v) { return v > 0.0 ? (v < 1.0 ? v : 1.0) : 0.0; }
 
 // Equilibrium, calculateEquilibriumDDF2Q9 (cellCalculations.cuh:32-52).
 template <class T>
@@ -99,7 +99,7 @@ inline void forcing_t(T ux, T uy, T gx, T gy, T* fi) {
 
 template <class T>
 inline T phi_of(T m, T rho) {  // calculatePhi (cellCalculations.cuh:153-158)
-  if (rho > 0.0f) return saturate(m / rho);
+  if (rho > 0.0f) return T(saturate(float(m / rho)));
   return 0.5f;
 }
 
