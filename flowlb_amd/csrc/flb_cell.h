@@ -341,7 +341,11 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
 #pragma unroll
       for (int i = 0; i < 9; i++) g[i] = R(omc) * g[i] + R(inv_tau) * fe[i];
     }
-  } else if (!forced) {  // TRT, magic parameter 1/4; the forced variant only streams
+  } else if (!forced || p.collision == 2) {
+    // TRT, magic parameter 1/4 (:475-510).  As written a forced TRT step only
+    // streams; collision 2 (no reference counterpart) defines the forced
+    // branch: Guo terms split into the pair-symmetric part, weighted 1 - w+/2,
+    // and the antisymmetric part, weighted 1 - w-/2 (SRT + Guo when w+ = w-).
     // The reference keeps these temporaries in float also for PRECISION = 64.
     const float ip = inv_tau;
     const float in = 1.0f / (0.25f / (1.0f / inv_tau - 0.5f) + 0.5f);
@@ -353,6 +357,16 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
       const float fea = float(fe[i] - fe[i + 1]);
       g[i] = g[i] - R(0.5f * ip * sym) - R(0.5f * in * (fa - fea));
       g[i + 1] = g[i + 1] - R(0.5f * ip * sym) - R(0.5f * in * (fea - fa));
+    }
+    if (forced) {
+      const R ap = R(1.0f - 0.5f * ip), an = R(1.0f - 0.5f * in);
+      g[0] = g[0] + ap * fi[0];
+#pragma unroll
+      for (int i = 1; i < 9; i += 2) {
+        const R fs = R(0.5f) * (fi[i] + fi[i + 1]), fd = R(0.5f) * (fi[i] - fi[i + 1]);
+        g[i] = g[i] + (ap * fs + an * fd);
+        g[i + 1] = g[i + 1] + (ap * fs - an * fd);
+      }
     }
   }
   store_outgoing<S, ODD>(p, nb, g);

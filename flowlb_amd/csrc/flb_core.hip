@@ -199,7 +199,7 @@ void initial_populations(uint32_t nx, uint32_t ny, const float* rho0, const floa
 int validate(const flb_options* o, uint32_t nx, uint32_t ny, std::string& why) {
   if (!(o->tau > 0.5f) || !std::isfinite(o->tau)) return why = "tau must be > 0.5", 1;
   if (o->storage < 0 || o->storage > 3) return why = "unknown storage format", 1;
-  if (o->collision < 0 || o->collision > 1) return why = "collision must be SRT or TRT", 1;
+  if (o->collision < 0 || o->collision > 2) return why = "collision must be SRT, TRT or TRT_GUO", 1;
   if (o->problem < 0 || o->problem > 1) return why = "problem must be monofluid or free surface", 1;
   if (o->mass_mode < 0 || o->mass_mode > 1) return why = "unknown mass mode", 1;
   if (o->block_x != 0 && (o->block_x < 64 || o->block_x > 1024 || o->block_x % 64))

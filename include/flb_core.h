@@ -60,8 +60,12 @@ enum {
 enum { FLB_STORAGE_FP32 = 0, FLB_STORAGE_FP16S = 1, FLB_STORAGE_BF16S = 2, FLB_STORAGE_FP64 = 3 };
 /* OptionsCalculation::typeProblem (src/io/reader.h:104). */
 enum { FLB_MONOFLUID = 0, FLB_FREE_SURFACE = 1 };
-/* OptionsCalculation::collisionOperator: 0 = SRT, 1 = TRT (src/cuda/lbmKernel2D.cu:444-511). */
-enum { FLB_SRT = 0, FLB_TRT = 1 };
+/* OptionsCalculation::collisionOperator: 0 = SRT, 1 = TRT (src/cuda/lbmKernel2D.cu:444-511).
+   As written the TRT step does not collide when a volumetric force is applied
+   (:485-488).  FLB_TRT_GUO (no reference counterpart) defines that branch:
+   TRT plus the Guo terms split into their pair-symmetric part, weighted
+   1 - w+/2, and antisymmetric part, weighted 1 - w-/2; unforced it is TRT. */
+enum { FLB_SRT = 0, FLB_TRT = 1, FLB_TRT_GUO = 2 };
 /* Excess-mass accounting of the free-surface passes.
    FLB_MASS_AS_WRITTEN: the reference's rules (receivers counted on the
    pre-pass flags, fresh own fill level in the exchange weight, fluid cells
@@ -77,7 +81,7 @@ typedef struct flb_options {
   float tau;                   /* relaxation time; 1/tau is formed in float */
   float fx, fy;                /* body force per cell (Guo forcing), lattice units */
   int32_t use_force;           /* USE_VOLUMETRIC_FORCE */
-  int32_t collision;           /* FLB_SRT / FLB_TRT */
+  int32_t collision;           /* FLB_SRT / FLB_TRT / FLB_TRT_GUO */
   int32_t smagorinsky;         /* SUBGRID_TURBULENCE_MODEL */
   int32_t problem;             /* FLB_MONOFLUID / FLB_FREE_SURFACE */
   int32_t storage;             /* FLB_STORAGE_* (PRECISION 32 -> FP32, 16 -> FP16S) */

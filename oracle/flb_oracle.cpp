@@ -341,10 +341,13 @@ struct Sim {
             for (int i = 0; i < 9; i++) lf[i] = feq[i];
           }
         } else {  // TRT (:475-510); with forcing the reference performs no collision
+          // collision 2 (no reference counterpart) defines the forced branch:
+          // the Guo terms split into the pair-symmetric part, weighted
+          // 1 - w+/2, and the antisymmetric part, weighted 1 - w-/2.
           const float ip = inv_tau;
           const float in = 1.0f / (0.25f / (1.0f / inv_tau - 0.5f) + 0.5f);
           if (!io) {
-            if (!forced) {
+            if (!forced || p.collision == 2) {
               lf[0] = lf[0] - ip * (lf[0] - feq[0]);
               for (int i = 1; i < 9; i += 2) {
                 const float fmf = float(lf[i] - feq[i] + lf[i + 1] - feq[i + 1]);
@@ -352,6 +355,15 @@ struct Sim {
                 const float fea = float(feq[i] - feq[i + 1]);
                 lf[i] = lf[i] - 0.5f * ip * fmf - 0.5f * in * (fa - fea);
                 lf[i + 1] = lf[i + 1] - 0.5f * ip * fmf - 0.5f * in * (fea - fa);
+              }
+              if (forced) {
+                const T ap = T(1.0f - 0.5f * ip), an = T(1.0f - 0.5f * in);
+                lf[0] = lf[0] + ap * fi[0];
+                for (int i = 1; i < 9; i += 2) {
+                  const T fs = T(0.5f) * (fi[i] + fi[i + 1]), fd = T(0.5f) * (fi[i] - fi[i + 1]);
+                  lf[i] = lf[i] + (ap * fs + an * fd);
+                  lf[i + 1] = lf[i + 1] + (ap * fs - an * fd);
+                }
               }
             }
           } else {

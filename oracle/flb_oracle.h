@@ -33,7 +33,7 @@ typedef struct orc_params {
   float inv_tau;          /* 1/tau kept as float (src/cuda/cudaInitData.cu:82) */
   float fx, fy;           /* body force, float (src/cuda/cudaInitData.cu:66-72) */
   int32_t use_force;      /* d_useVolumetricForce */
-  int32_t collision;      /* 0 = SRT, 1 = TRT */
+  int32_t collision;      /* 0 = SRT, 1 = TRT, 2 = TRT with Guo forcing */
   int32_t smagorinsky;    /* d_useSubgridModel */
   int32_t free_surface;   /* TYPE_PROBLEM = FREE_SURFACE */
   int32_t gas_check_corrected; /* 0: neighbour test of storeReconstructedGasf as written */

@@ -115,3 +115,21 @@ def test_fp64_not_offered_for_adjoint_and_batch():
     inst = cases.batch_instances(2, seed=1)
     with pytest.raises(core.FlbError):
         core.Batch(inst, storage="fp64")
+
+
+@pytest.mark.parametrize("storage,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
+def test_forced_trt_branch(orc, storage, tol):
+    """FLB_TRT_GUO (collision 2, the defined forced TRT branch) on the forced
+    256 x 64 culvert against the oracle's restatement."""
+    case = cases.scaled(cases.ADJOINT, 4)
+    flags, rho0, u0 = case.build()
+    s = core.Solver(flags, case.nx, case.ny, case.options(storage, collision=2), rho0=rho0, u0=u0)
+    o = oracle(orc, case, flags, rho0, u0, collision=2)
+    with s:
+        s.step(1000)
+        g = s.read64()
+    o.step(1000)
+    r = o.get()
+    eu = rel_l2(g["u"], r["u"])
+    print(f"\n{storage} forced TRT 256x64 x 1000: relL2 u {eu:.2e}")
+    assert eu < tol

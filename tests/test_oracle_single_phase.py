@@ -233,6 +233,47 @@ def test_trt_with_force_streams_only(orc):
         np.testing.assert_array_equal(post[k, dst], pre[k])
 
 
+def test_trt_guo_equals_srt_guo_at_unit_tau(orc):
+    """The defined forced TRT branch (collision 2): at tau = 1 both rates are 1,
+    so the symmetric/antisymmetric split of the Guo terms recombines to SRT's
+    (1 - w/2) F_i.  Without a force collision 2 is plain TRT."""
+    nx, ny = 10, 8
+    N = nx * ny
+    flags = channel(orc, nx, ny)
+    rng = np.random.default_rng(9)
+    kw = dict(rho0=1 + rng.uniform(-1e-3, 1e-3, N), u0=inlet_u(flags, N, 0.03), threads=1)
+    a = orc.Oracle(flags, nx, ny, 1.0, collision=0, use_force=True, fx=2e-5, fy=-1e-5, **kw)
+    b = orc.Oracle(flags, nx, ny, 1.0, collision=2, use_force=True, fx=2e-5, fy=-1e-5, **kw)
+    a.step(30)
+    b.step(30)
+    np.testing.assert_allclose(a.get()["f"], b.get()["f"], rtol=0, atol=1e-8)
+    c = orc.Oracle(flags, nx, ny, 0.7, collision=1, **kw)
+    d = orc.Oracle(flags, nx, ny, 0.7, collision=2, **kw)
+    c.step(30)
+    d.step(30)
+    np.testing.assert_array_equal(c.get()["f"], d.get()["f"])
+
+
+def test_trt_guo_poiseuille_profile(orc):
+    """Forced TRT (collision 2) reproduces the body-force Poiseuille parabola
+    at tau = 0.8, where the as-written TRT would not collide at all."""
+    nx, ny = 4, 18
+    N = nx * ny
+    flags = channel(orc, nx, ny, left=FLUID, right=FLUID)
+    tau = 0.8
+    nu = (tau - 0.5) / 3
+    F = 1e-6
+    o = orc.Oracle(flags, nx, ny, tau, use_force=True, fx=F, collision=2, threads=1)
+    o.step(20000)
+    ux = o.get()["u"][:N].reshape(ny, nx)[:, 0]
+    H = ny - 2
+    y = np.arange(ny) - 0.5
+    exact = float(np.float32(F)) / (2 * nu) * y * (H - y)
+    fluid = slice(1, ny - 1)
+    err = np.linalg.norm(ux[fluid] - exact[fluid]) / np.linalg.norm(exact[fluid])
+    assert err < 5e-3, err
+
+
 def test_smagorinsky_effective_tau(orc):
     """tau_eff = (tau + sqrt(tau^2 + 18 (C Delta)^2 sqrt(Q)/rho))/2 with
     18 (C Delta)^2 = 0.54037965; at rest Q = 0 and tau_eff = tau."""
