@@ -42,11 +42,13 @@ __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
   const uint32_t n = c.n;
   const uint64_t N = p.N;
   const uint8_t fl = p.flags[n];
+  // Most wavefronts of a free-surface domain are all gas: they leave after one
+  // byte load, before any neighbour is touched.
+  if (fl & (kGas | kWall | kInlet)) return;
   uint32_t nb[9];
   neighbours(p, c.x, c.y, nb);
   uint8_t nf[9];
-  neighbour_flags(p, nb, nf);  // issued with the own flag: one round trip
-  if (fl & (kGas | kWall | kInlet)) return;
+  neighbour_flags(p, nb, nf);
   const bool cons = p.mass_mode != 0;
   // Conservative mode: the mass of a fluid cell whose eight neighbours are all
   // fluid or interface is updated by stream-collide instead (interior_fluid).
