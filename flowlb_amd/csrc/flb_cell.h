@@ -199,14 +199,18 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   // Free surface: every load that depends only on the own flag is issued here,
   // with the populations, so that the cell waits for memory once.
   uint8_t nf[9];
-  R ex[9], rc[9], m_if = R(0);
+  R ex[9], rc[9], m_if = R(0), resid = R(0);
   const bool cons = FS && p.mass_mode != 0;
   const bool fluid_dev = cons && (fl & (kFGI | kInlet | kOutlet)) == kFluid;
   const bool iface = FS && (fl & kFGI) == kInterface;
+  // 16-bit storage rounds every stored population; that rounding changes the
+  // population mass, which the interior-fluid shortcut below would take over.
+  constexpr bool kRounded = S::kBytes < 4;
   if (FS) neighbour_flags(p, nb, nf);
   if (fluid_dev) {
 #pragma unroll
     for (int i = 1; i < 9; i++) ex[i] = fld<R>(p.excess)[nb[i]];
+    if (kRounded) resid = fld<R>(p.rec)[n];  // plane 0: last step's residual
   }
   if (iface) {
     m_if = fld<R>(p.mass)[n];
@@ -258,10 +262,18 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
       p.trace[2 * N + n] = my;
     }
   }
-  if (fluid_dev && interior_fluid(fl, nf)) {  // deviation = pulled shares
+  if (fluid_dev && interior_fluid(fl, nf)) {
+    // Deviation m - rho that link-by-link accounting would give: the pulled
+    // shares, plus the rounding of rho itself, plus (16-bit storage) the mass
+    // the previous step's stores rounded away.
     double m = 0.0;
 #pragma unroll
     for (int i = 1; i < 9; i++) m += double(ex[i]);
+    double sg = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) sg += double(g[i]);
+    m += (1.0 + sg) - double(rho);
+    if (kRounded) m += double(resid);
     fld<R>(p.mass)[n] = R(m);
   }
   if (iface) {  // interface -> fluid / gas test (:353-367)
@@ -376,6 +388,14 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
         g[i + 1] = g[i + 1] + (ap * fs - an * fd);
       }
     }
+  }
+  if (kRounded && cons && !(fl & kInlet)) {
+    // Residual of this step's collision and storage rounding, rho - 1 - sum of
+    // the stored populations, for the next step's interior-fluid deviation.
+    double so = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) so += double(S::round(g[i]));
+    fld<R>(p.rec)[n] = R((double(rho) - 1.0) - so);
   }
   store_outgoing<S, ODD>(p, nb, g);
 }

@@ -380,6 +380,7 @@ int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const 
   int rc;
   if ((rc = upload_field(h, h->mass, mass, n))) return rc;
   if ((rc = upload_field(h, h->phi, phi, n))) return rc;
+  if ((rc = zero_field(h, h->rec, n))) return rc;  // stream-collide's mass residuals
   if (excess) return upload_field(h, h->excess, excess, n);
   return zero_field(h, h->excess, n);
 }

@@ -45,6 +45,7 @@ struct StoreF32 {
   static constexpr int kBytes = 4;
   __device__ static float load(const T* p, size_t i) { return p[i]; }
   __device__ static void store(T* p, size_t i, float v) { p[i] = v; }
+  __device__ static float round(float v) { return v; }  // value a store keeps
 };
 
 struct StoreF64 {
@@ -53,6 +54,7 @@ struct StoreF64 {
   static constexpr int kBytes = 8;
   __device__ static double load(const T* p, size_t i) { return p[i]; }
   __device__ static void store(T* p, size_t i, double v) { p[i] = v; }
+  __device__ static double round(double v) { return v; }
 };
 
 // IEEE binary16, scaled by 2^15 so that the shifted populations (|f - w| < 1)
@@ -63,6 +65,9 @@ struct StoreF16S {
   static constexpr int kBytes = 2;
   __device__ static float load(const T* p, size_t i) { return float(p[i]) * 3.0517578125e-05f; }
   __device__ static void store(T* p, size_t i, float v) { p[i] = _Float16(v * 32768.0f); }
+  __device__ static float round(float v) {
+    return float(_Float16(v * 32768.0f)) * 3.0517578125e-05f;
+  }
 };
 
 // bfloat16 (upper half of an FP32), round to nearest even on store.
@@ -77,6 +82,11 @@ struct StoreBF16S {
     uint32_t u = __float_as_uint(v);
     u += 0x7fffu + ((u >> 16) & 1u);
     p[i] = uint16_t(u >> 16);
+  }
+  __device__ static float round(float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
   }
 };
 
@@ -117,11 +127,12 @@ __device__ __forceinline__ R* fld(float* p) {
 }
 
 // Block -> tile mapping.  Workgroups are dispatched round-robin over the 8
-// XCDs; remapping gives each XCD a contiguous range of tiles (rows), so the
-// row-shifted loads and stores of neighbouring tiles meet in one L2.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
-  const uint32_t q = nb >> 3, r = nb & 7u, xcd = b & 7u, k = b >> 3;
-  return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
-}
+// XCDs (block b to XCD b mod 8), each with its own L2.  Tiles are numbered row
+// by row and taken in that order: with a row of a multiple of 8 tiles the tiles
+// above and below a tile run on the same XCD, so the row-shifted loads meet in
+// its L2, and every XCD gets an equal share of every band of rows.  (Giving
+// each XCD a contiguous band of rows instead left the XCDs holding the gas
+// rows of a free-surface domain idle while the others worked on the water.)
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t) { return b; }
 
 }  // namespace flb

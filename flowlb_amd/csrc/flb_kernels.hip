@@ -182,9 +182,12 @@ constexpr int kTileW = kFsTileW, kTileH = kFsTileH, kHalo = 3;
 constexpr int kLdsW = kTileW + 2 * kHalo, kLdsH = kTileH + 2 * kHalo;
 constexpr int kUpdateThreads = kTileW * kTileH;
 
+// Periodic index for the halo, v in [-kHalo, n + kTileW + kHalo): a compare
+// and add/subtract (a loop only for domains narrower than the tile).
 __device__ __forceinline__ int wrap(int v, int n) {
-  int w = v % n;
-  return w < 0 ? w + n : w;
+  while (v < 0) v += n;
+  while (v >= n) v -= n;
+  return v;
 }
 
 // LDS offsets of the eight neighbours, in the order of neighbours().

@@ -4,9 +4,13 @@ float/double arithmetic.  With both sides in double the comparison is tight:
 the two differ only by the shifted storage, fused multiply-adds and summation
 order (~1e-16 per operation).  Bars: populations within 1e-13 after a few
 steps; density and velocity within 1e-10 relative L2 over 1500 steps of the
-512 x 128 culvert (SRT, TRT and Smagorinsky); free-surface flags identical
-and mass within 1e-10 after 400 steps; conservative-mode drift below 1e-12
-per 10^4 steps."""
+512 x 128 culvert for SRT and Smagorinsky.  Where the reference keeps float
+temporaries inside the double path -- TRT's pair sums, the free surface's
+fill level (__saturatef) and neighbour averages -- a 1e-17 difference of their
+double inputs can round to a different float, so those runs agree to float
+precision only: TRT within 1e-7 relative L2, free-surface velocity within 1e-7
+and mass within 1e-6 (flags identical) after 400 steps.  Conservative-mode
+drift below 1e-12 per 10^4 steps."""
 import dataclasses
 
 import numpy as np
@@ -50,9 +54,10 @@ def test_slots_after_few_steps(orc, steps):
     np.testing.assert_allclose(g["rho"], r["rho"], rtol=0, atol=1e-13)
 
 
-@pytest.mark.parametrize("variant", [dict(), dict(collision=1), dict(smagorinsky=1)],
+@pytest.mark.parametrize("variant,tol", [(dict(), 1e-10), (dict(collision=1), 1e-7),
+                                         (dict(smagorinsky=1), 1e-10)],
                          ids=["srt", "trt", "smagorinsky"])
-def test_culvert_1500_steps(orc, variant):
+def test_culvert_1500_steps(orc, variant, tol):
     flags, rho0, u0 = SP.build()
     s = core.Solver(flags, SP.nx, SP.ny, SP.options("fp64", **variant), rho0=rho0, u0=u0)
     o = oracle(orc, SP, flags, rho0, u0, collision=variant.get("collision", 0),
@@ -64,7 +69,7 @@ def test_culvert_1500_steps(orc, variant):
     r = o.get()
     eu, er = rel_l2(g["u"], r["u"]), rel_l2(g["rho"], r["rho"])
     print(f"\nfp64 512x128 x 1500 {variant}: relL2 u {eu:.2e} rho {er:.2e}")
-    assert eu < 1e-10 and er < 1e-10
+    assert eu < tol and er < tol
 
 
 def test_free_surface_400_steps(orc):
@@ -82,7 +87,7 @@ def test_free_surface_400_steps(orc):
           f"{(g['flags'] == r['flags']).mean():.6f}, max|dm| {dm:.2e} max|du| {du:.2e}")
     assert moved > 100
     np.testing.assert_array_equal(g["flags"], r["flags"])
-    assert dm < 1e-10 and du < 1e-10
+    assert dm < 1e-6 and du < 1e-7
 
 
 def test_conservation_1e4_steps():
@@ -112,12 +117,13 @@ def test_fp64_not_offered_for_adjoint_and_batch():
     with case.solver("fp64") as s:
         with pytest.raises(core.FlbError):
             s.adjoint(10, case.nx - 2, 5, 9, ckpt=5)
-    inst = cases.batch_instances(2, seed=1)
+    c = cases.scaled(cases.batch_instances(1, seed=1)[0], 4)
+    fl, r0, v0 = c.build()
     with pytest.raises(core.FlbError):
-        core.Batch(inst, storage="fp64")
+        core.Batch([dict(nx=c.nx, ny=c.ny, tau=c.tau, flags=fl, rho0=r0, u0=v0)], storage="fp64")
 
 
-@pytest.mark.parametrize("storage,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
+@pytest.mark.parametrize("storage,tol", [("fp64", 1e-7), ("fp32", 1e-5)])
 def test_forced_trt_branch(orc, storage, tol):
     """FLB_TRT_GUO (collision 2, the defined forced TRT branch) on the forced
     256 x 64 culvert against the oracle's restatement."""
