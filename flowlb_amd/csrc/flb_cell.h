@@ -56,11 +56,15 @@ struct Cell {
   bool valid;
 };
 
-__device__ __forceinline__ Cell locate(const Params& p) {
-  const uint32_t L = xcd_tile(blockIdx.x, p.nblocks);
+// Cell of this thread in row-segment tile L (tiles numbered row by row).
+__device__ __forceinline__ Cell locate_tile(const Params& p, uint32_t L) {
   const uint32_t y = L / p.tiles_x;
   const uint32_t x = (L - y * p.tiles_x) * blockDim.x + threadIdx.x;
   return Cell{x, y, x + y * p.nx, x < p.nx};
+}
+
+__device__ __forceinline__ Cell locate(const Params& p) {
+  return locate_tile(p, xcd_tile(blockIdx.x, p.nblocks));
 }
 
 // Periodic neighbour indices (src/cuda/cellCalculations.cuh:82-109) without

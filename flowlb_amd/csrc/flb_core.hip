@@ -408,7 +408,7 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
     flb::Params q = p;  // passes 2-4: one block per kFsTileW x kFsTileH tile
     q.tiles_x = (h->nx + flb::kFsTileW - 1) / flb::kFsTileW;
     q.nblocks = q.tiles_x * ((h->ny + flb::kFsTileH - 1) / flb::kFsTileH);
-    if ((rc = run(FLB_K_FS_UPDATE, flb::get_fs_update(st, odd), q, flb::kFsTileW * flb::kFsTileH)))
+    if ((rc = run(FLB_K_FS_UPDATE, flb::get_fs_update(st, odd, h->nx, h->ny), q, flb::kFsTileW * flb::kFsTileH)))
       return rc;
     h->cur ^= 1;  // the update wrote the other flag buffer
   } else {

@@ -1,19 +1,22 @@
 """Accuracy of the storage formats on the free-surface dam break, measured on
-three physical quantities: velocity (relative L2), the water-surface profile
-(per column, the height where the fill level crosses 0.5) and the discharge
-through the culvert barrel, sum over the barrel column of phi rho u_x,
-time-averaged.
+physical quantities: the discharge through the culvert barrel (sum over the
+barrel column of phi rho u_x, time-averaged), the water-surface profile (per
+column, the height where the fill level crosses 0.5) and the velocity.
 
 * 1024 x 256 (the 4096 x 1024 case / 4): FP32, FP16S and BF16S storage on the
-  GPU against the FP64 oracle at the same size;
-* 8192 x 2048: FP16S and BF16S against FP32 on the GPU, velocity box-averaged
-  over 8 x 8 cells, surface heights over 8 columns.
+  GPU against the FP64 oracle at the same size, 4000 steps;
+* 8192 x 2048: FP16S and BF16S against FP32 on the GPU, 8000 steps, surface
+  heights averaged over 8 columns.
 
 The runs use the corrected reconstruction gate (gas_check_corrected = 1): with
 the reference's gate the water column of this case barely moves (its front
-advances ~2 cells per 1000 steps) and there is no flow to measure.  Bars
-(stated per format below) bound what the test measures; the measured values
-are printed."""
+advances ~2 cells per 1000 steps) and there is no flow to measure.  At
+tau = 0.51 the breaking wave is chaotic: any rounding difference decorrelates
+the instantaneous fields (FP32 and FP64 differ by ~100 cells in the local
+surface height and ~30 % in velocity after 4000 steps), so the test bounds the
+statistics that survive that -- the time-averaged discharge and the mean
+surface-height difference -- and prints the rest.  A storage format passes
+when it stays as close to the reference run as FP32 does."""
 import numpy as np
 import pytest
 
@@ -22,7 +25,8 @@ from flowlb_amd import cases, core
 pytestmark = pytest.mark.gpu
 
 SMALL = cases.FS_1024
-STEPS, EVERY, T_AVG = 4000, 50, 2000
+STEPS, EVERY, T_AVG = 4000, 50, 2000          # 1024 x 256
+BIG_STEPS, BIG_EVERY, BIG_T_AVG = 8000, 500, 4000   # 8192 x 2048
 
 
 def surface(phi, nx, ny):
@@ -51,12 +55,12 @@ def discharge(fields, flags, nx, ny, column):
     return float(np.sum(np.where(wet, phi * rho * ux, 0.0)))
 
 
-def run_gpu(case, storage, column, every=EVERY):
+def run_gpu(case, storage, column, steps=STEPS, every=EVERY, t_avg=T_AVG):
     qs = []
     with case.solver(storage, mass_mode=1, gas_check_corrected=1) as s:
-        for t in range(every, STEPS + 1, every):
+        for t in range(every, steps + 1, every):
             s.step(every)
-            if t > STEPS - T_AVG:
+            if t > steps - t_avg:
                 g = s.read()
                 qs.append(discharge(g, g["flags"], case.nx, case.ny, column))
         g = s.read()
@@ -83,9 +87,9 @@ def reference_small(orc):
     return run_oracle(orc, SMALL, column), column
 
 
-@pytest.mark.parametrize("storage,max_dh,max_dq,max_du", [
-    ("fp32", 2.0, 1e-2, 5e-2), ("fp16s", 4.0, 5e-2, 1e-1), ("bf16s", 16.0, 3e-1, 1.0)])
-def test_small_against_fp64_oracle(reference_small, storage, max_dh, max_dq, max_du):
+@pytest.mark.parametrize("storage,max_mean_dh,max_dq", [
+    ("fp32", 1.5, 3e-3), ("fp16s", 1.5, 3e-3), ("bf16s", 2.0, 5e-3)])
+def test_small_against_fp64_oracle(reference_small, storage, max_mean_dh, max_dq):
     (r, q_ref), column = reference_small
     g, q = run_gpu(SMALL, storage, column)
     nx, ny = SMALL.nx, SMALL.ny
@@ -95,8 +99,9 @@ def test_small_against_fp64_oracle(reference_small, storage, max_dh, max_dq, max
     print(f"\n{storage} 1024x256 x {STEPS} vs FP64 oracle: surface max|dh| {dh.max():.3f} "
           f"mean {dh.mean():.4f} cells; discharge {q:.5e} vs {q_ref:.5e} (rel {dq:.2e}); "
           f"relL2 u {du:.2e}")
-    assert abs(q_ref) > 0
-    assert dh.max() <= max_dh and dq <= max_dq and du <= max_du
+    assert abs(q_ref) > 0.5             # the barrel flows
+    assert dh.mean() <= max_mean_dh and dq <= max_dq
+    assert du < 1.0                     # decorrelated, not blown up
 
 
 def box(a, nx, ny, k):
@@ -106,19 +111,21 @@ def box(a, nx, ny, k):
 def test_large_sixteen_bit_against_fp32():
     case = cases.FS_8192
     column = (case.emb[0] + case.emb[1]) // 2
-    out = {st: run_gpu(case, st, column, every=200) for st in ("fp32", "fp16s", "bf16s")}
+    out = {st: run_gpu(case, st, column, BIG_STEPS, BIG_EVERY, BIG_T_AVG)
+           for st in ("fp32", "fp16s", "bf16s")}
     nx, ny = case.nx, case.ny
     g32, q32 = out["fp32"]
     ux32 = box(g32["u"][:nx * ny], nx, ny, 8)
     h32 = surface(g32["phi"], nx, ny).reshape(-1, 8).mean(axis=1)
-    for st, (max_dh, max_dq, max_du) in {"fp16s": (4.0, 5e-2, 1e-1),
-                                          "bf16s": (16.0, 3e-1, 1.0)}.items():
+    assert abs(q32) > 0.5
+    for st, (max_mean_dh, max_dq) in {"fp16s": (2.0, 5e-3), "bf16s": (3.0, 1e-2)}.items():
         g, q = out[st]
         ux = box(g["u"][:nx * ny], nx, ny, 8)
         h = surface(g["phi"], nx, ny).reshape(-1, 8).mean(axis=1)
         du = float(np.linalg.norm(ux - ux32) / np.linalg.norm(ux32))
         dh = np.abs(h - h32)
         dq = abs(q - q32) / abs(q32)
-        print(f"\n{st} 8192x2048 x {STEPS} vs FP32: surface (8-column mean) max|dh| {dh.max():.3f} "
-              f"cells; discharge rel {dq:.2e}; box-averaged relL2 u_x {du:.2e}")
-        assert dh.max() <= max_dh and dq <= max_dq and du <= max_du
+        print(f"\n{st} 8192x2048 x {BIG_STEPS} vs FP32: surface (8-column mean) max|dh| "
+              f"{dh.max():.3f} mean {dh.mean():.3f} cells; discharge {q:.5e} vs {q32:.5e} "
+              f"(rel {dq:.2e}); box-averaged relL2 u_x {du:.2e}")
+        assert dh.mean() <= max_mean_dh and dq <= max_dq
