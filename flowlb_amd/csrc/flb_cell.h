@@ -311,10 +311,17 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     forced = true;
   }
   if (MOM || FS) {
+    // Free surface: every cell's density is read by pass 4, but the velocity
+    // of a fluid cell only at readback -- pass 1 reads the velocity of
+    // inlet/outlet interface cells, pass 3 averages interface neighbours
+    // (a fluid cell never neighbours a gas or gas->interface cell).
+    const bool want_u = MOM || (fl & kFGI) != kFluid;
     if (!io) {
       rho_f[n] = rho;
-      u_f[n] = ux;
-      u_f[N + n] = uy;
+      if (want_u) {
+        u_f[n] = ux;
+        u_f[N + n] = uy;
+      }
     } else if (fl & kOutlet) {
       u_f[n] = ux;
       u_f[N + n] = uy;

@@ -403,7 +403,7 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
   int rc;
   if (fs) {
     if ((rc = run(FLB_K_FS1, flb::get_fs1(st, odd), p, bx))) return rc;
-    if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, true), p, bx)))
+    if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, moments), p, bx)))
       return rc;
     flb::Params q = p;  // passes 2-4: one block per kFsTileW x kFsTileH tile
     q.tiles_x = (h->nx + flb::kFsTileW - 1) / flb::kFsTileW;

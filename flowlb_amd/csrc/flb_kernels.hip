@@ -518,6 +518,7 @@ using HistFn = void (*)(const Params, unsigned long long*);
 
 template <class S>
 KernelFn stream_collide_fn(int odd, bool fs, bool mom) {
+  if (fs && mom) return odd ? k_stream_collide<S, 1, true, true> : k_stream_collide<S, 0, true, true>;
   if (fs) return odd ? k_stream_collide<S, 1, true, false> : k_stream_collide<S, 0, true, false>;
   if (mom) return odd ? k_stream_collide<S, 1, false, true> : k_stream_collide<S, 0, false, true>;
   return odd ? k_stream_collide<S, 1, false, false> : k_stream_collide<S, 0, false, false>;
