@@ -1,22 +1,30 @@
 """Accuracy of the storage formats on the free-surface dam break, measured on
 physical quantities: the discharge through the culvert barrel (sum over the
 barrel column of phi rho u_x, time-averaged), the water-surface profile (per
-column, the height where the fill level crosses 0.5) and the velocity.
+column, the height where the fill level crosses 0.5), the position of the
+surge front and the velocity.
 
 * 1024 x 256 (the 4096 x 1024 case / 4): FP32, FP16S and BF16S storage on the
   GPU against the FP64 oracle at the same size, 4000 steps;
-* 8192 x 2048: FP16S and BF16S against FP32 on the GPU, 8000 steps, surface
-  heights averaged over 8 columns.
+* 8192 x 2048: FP16S and BF16S against FP32 on the GPU over the first 3000
+  steps -- the collapse of the 1200-cell water column up to the moment the
+  surge reaches the embankment -- comparing front position, surface heights
+  averaged over 8 columns and velocity averaged over 8 x 8 boxes.  Past that
+  moment the surge meets the wall at lattice speeds near 0.3, which SRT at
+  tau = 0.51 does not survive (the FP32 run turns to NaN between steps 4000
+  and 5000), so the barrel discharge is measured on the small
+  case only.
 
 The runs use the corrected reconstruction gate (gas_check_corrected = 1): with
 the reference's gate the water column of this case barely moves (its front
 advances ~2 cells per 1000 steps) and there is no flow to measure.  At
-tau = 0.51 the breaking wave is chaotic: any rounding difference decorrelates
-the instantaneous fields (FP32 and FP64 differ by ~100 cells in the local
-surface height and ~30 % in velocity after 4000 steps), so the test bounds the
-statistics that survive that -- the time-averaged discharge and the mean
-surface-height difference -- and prints the rest.  A storage format passes
-when it stays as close to the reference run as FP32 does."""
+tau = 0.51 the breaking wave of the small case is chaotic: any rounding
+difference decorrelates the instantaneous fields (FP32 and FP64 differ by
+~100 cells in the local surface height and ~30 % in velocity after 4000
+steps), so that test bounds the statistics that survive -- the time-averaged
+discharge and the mean surface-height difference -- and prints the rest.  A
+storage format passes when it stays as close to the reference run as FP32
+does."""
 import numpy as np
 import pytest
 
@@ -26,7 +34,7 @@ pytestmark = pytest.mark.gpu
 
 SMALL = cases.FS_1024
 STEPS, EVERY, T_AVG = 4000, 50, 2000          # 1024 x 256
-BIG_STEPS, BIG_EVERY, BIG_T_AVG = 8000, 500, 4000   # 8192 x 2048
+BIG_STEPS = 3000                                    # 8192 x 2048
 
 
 def surface(phi, nx, ny):
@@ -108,24 +116,37 @@ def box(a, nx, ny, k):
     return a.reshape(ny // k, k, nx // k, k).mean(axis=(1, 3))
 
 
+def front(flags, nx, ny):
+    """Largest x of a wet column (the surge front)."""
+    wet = (flags.reshape(ny, nx) & (core.FLUID | core.INTERFACE)) != 0
+    return int(np.nonzero(wet.any(axis=0))[0].max())
+
+
 def test_large_sixteen_bit_against_fp32():
     case = cases.FS_8192
-    column = (case.emb[0] + case.emb[1]) // 2
-    out = {st: run_gpu(case, st, column, BIG_STEPS, BIG_EVERY, BIG_T_AVG)
-           for st in ("fp32", "fp16s", "bf16s")}
     nx, ny = case.nx, case.ny
-    g32, q32 = out["fp32"]
+    out = {}
+    for st in ("fp32", "fp16s", "bf16s"):
+        with case.solver(st, mass_mode=1, gas_check_corrected=1) as s:
+            m0 = s.mass()
+            s.step(BIG_STEPS)
+            out[st] = (s.read(), abs(s.mass() - m0) / m0)
+    g32, _ = out["fp32"]
     ux32 = box(g32["u"][:nx * ny], nx, ny, 8)
     h32 = surface(g32["phi"], nx, ny).reshape(-1, 8).mean(axis=1)
-    assert abs(q32) > 0.5
-    for st, (max_mean_dh, max_dq) in {"fp16s": (2.0, 5e-3), "bf16s": (3.0, 1e-2)}.items():
-        g, q = out[st]
+    x32 = front(g32["flags"], nx, ny)
+    assert x32 > case.water[0] + 100    # the column has collapsed
+    for st, (max_dx, max_mean_dh, max_du) in {"fp16s": (4, 0.5, 2e-2),
+                                              "bf16s": (16, 2.0, 1e-1)}.items():
+        g, drift = out[st]
         ux = box(g["u"][:nx * ny], nx, ny, 8)
         h = surface(g["phi"], nx, ny).reshape(-1, 8).mean(axis=1)
         du = float(np.linalg.norm(ux - ux32) / np.linalg.norm(ux32))
         dh = np.abs(h - h32)
-        dq = abs(q - q32) / abs(q32)
-        print(f"\n{st} 8192x2048 x {BIG_STEPS} vs FP32: surface (8-column mean) max|dh| "
-              f"{dh.max():.3f} mean {dh.mean():.3f} cells; discharge {q:.5e} vs {q32:.5e} "
-              f"(rel {dq:.2e}); box-averaged relL2 u_x {du:.2e}")
-        assert dh.mean() <= max_mean_dh and dq <= max_dq
+        x = front(g["flags"], nx, ny)
+        dx = abs(x - x32)
+        print(f"\n{st} 8192x2048 x {BIG_STEPS} vs FP32: front {x} vs {x32}; "
+              f"surface (8-column mean) max|dh| {dh.max():.3f} mean {dh.mean():.4f} cells; "
+              f"box-averaged relL2 u_x {du:.2e}; mass drift {drift:.2e} (fp32 {out['fp32'][1]:.2e})")
+        assert dx <= max_dx and dh.mean() <= max_mean_dh and du <= max_du
+        assert drift <= 1e-6
