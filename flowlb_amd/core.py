@@ -30,7 +30,7 @@ class Options(C.Structure):
                 ("use_force", C.c_int32), ("collision", C.c_int32), ("smagorinsky", C.c_int32),
                 ("problem", C.c_int32), ("storage", C.c_int32),
                 ("gas_check_corrected", C.c_int32), ("mass_mode", C.c_int32),
-                ("block_x", C.c_int32), ("device", C.c_int32)]
+                ("block_x", C.c_int32), ("device", C.c_int32), ("row_pitch", C.c_int32)]
 
 
 class Rect(C.Structure):
@@ -161,6 +161,7 @@ PROTOTYPES = [
                                C.c_double, C.c_int32]),
     ("flb_get_shape", C.c_int, [_H, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_int32)]),
+    ("flb_get_layout", C.c_int, [_H, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     ("flb_build_channel_flags", C.c_int, [C.POINTER(Channel), _U8]),
     ("flb_seed_free_surface", C.c_int, [_U8, C.c_uint32, C.c_uint32]),
     ("flb_init_free_surface_fields", C.c_int, [_U8, C.c_size_t, _F, _F, _F]),
@@ -279,6 +280,13 @@ class Solver:
     @property
     def time(self):
         return lib().flb_time(self.h)
+
+    @property
+    def layout(self):
+        """(row pitch, plane size) of the device fields (flb_get_layout)."""
+        pitch, plane = C.c_uint32(), C.c_uint64()
+        check(lib().flb_get_layout(self.h, C.byref(pitch), C.byref(plane)), self.h, "flb_get_layout")
+        return pitch.value, plane.value
 
     # -- readback
     def read(self):

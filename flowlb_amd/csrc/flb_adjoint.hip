@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "flb_cell.h"
 #include "flb_internal.h"
 
 namespace flb {
@@ -55,7 +56,7 @@ __device__ __forceinline__ uint32_t cell_neighbour(const Params& p, uint32_t x, 
   if (cx(i) == -1) xx = x == 0 ? nx - 1 : x - 1;
   if (cy(i) == 1) yy = y + 1 == ny ? 0 : y + 1;
   if (cy(i) == -1) yy = y == 0 ? ny - 1 : y - 1;
-  return xx + yy * nx;
+  return xx + yy * p.pitch;
 }
 
 // Slot (plane, cell) where the forward step of parity ODD stores population i.
@@ -91,12 +92,12 @@ __device__ __forceinline__ void block_add(double v, double* target) {
 
 template <int ODD>
 __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjParams a) {
-  const uint32_t L = xcd_tile(blockIdx.x, p.nblocks);
-  const uint32_t y = L / p.tiles_x;
-  const uint32_t x = (L - y * p.tiles_x) * blockDim.x + threadIdx.x;
+  uint32_t tx, y;
+  block_tile(blockIdx.x, p.tiles_x, p.ny, tx, y);
+  const uint32_t x = tx * blockDim.x + threadIdx.x;
   double duin = 0.0;
   if (x < p.nx) {
-    const uint32_t n = x + y * p.nx;
+    const uint32_t n = x + y * p.pitch;
     const uint64_t N = p.N;
     const uint8_t fl = p.flags[n];
     if (!(fl & (kGas | kWall))) {
@@ -107,10 +108,11 @@ __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjP
       float v[9];
 #pragma unroll
       for (int i = 0; i < 9; i++) v[i] = a.abar[store_index<ODD>(p, n, nb, i)];
-      const uint8_t fl_prev = p.flags[n > 0 ? n - 1 : n];  // unconditional: one round trip
-      const uint8_t fl_next = p.flags[n + 1 < N ? n + 1 : n];
-      const bool pub_r = n > 0 && (fl_prev & kInlet);
-      const bool pub_v = n + 1 < N && (fl_next & kOutlet) && !(fl & kInlet);
+      const Linear lin = linear_neighbours(p, x, y, n);
+      const uint8_t fl_prev = p.flags[lin.prev];  // unconditional: one round trip
+      const uint8_t fl_next = p.flags[lin.next];
+      const bool pub_r = lin.has_prev && (fl_prev & kInlet);
+      const bool pub_v = lin.has_next && (fl_next & kOutlet) && !(fl & kInlet);
       float rb = 0.0f, ubx = 0.0f, uby = 0.0f;
       if (pub_r) {
         rb = a.rbar[ODD][n];
@@ -141,11 +143,11 @@ __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjP
         }
         rb += -(ubx * Fx + uby * Fy) * 0.5f / (rho * rho);
         if (io & kInlet) {
-          a.rbar[ODD ^ 1][n + 1] += rb;
+          a.rbar[ODD ^ 1][lin.next] += rb;
           duin = ubx;
         } else {
-          a.ubar[ODD ^ 1][n - 1] += ubx;
-          a.ubar[ODD ^ 1][N + n - 1] += uby;
+          a.ubar[ODD ^ 1][lin.prev] += ubx;
+          a.ubar[ODD ^ 1][N + lin.prev] += uby;
         }
       } else {
         const float rho = t0, mx = t1, my = t2;
@@ -201,12 +203,12 @@ __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjP
 // The initial populations of inlet cells, feq(rho0, u0) stored with the odd
 // pattern, depend on the inlet velocity.
 __global__ void __launch_bounds__(256) k_adjoint_inlet_init(const Params p, const AdjParams a) {
-  const uint32_t L = xcd_tile(blockIdx.x, p.nblocks);
-  const uint32_t y = L / p.tiles_x;
-  const uint32_t x = (L - y * p.tiles_x) * blockDim.x + threadIdx.x;
+  uint32_t tx, y;
+  block_tile(blockIdx.x, p.tiles_x, p.ny, tx, y);
+  const uint32_t x = tx * blockDim.x + threadIdx.x;
   double acc = 0.0;
   if (x < p.nx) {
-    const uint32_t n = x + y * p.nx;
+    const uint32_t n = x + y * p.pitch;
     const uint8_t fl = p.flags[n];
     if ((fl & kInlet) && !(fl & (kGas | kWall))) {
       uint32_t nb[9];
@@ -229,7 +231,7 @@ __global__ void __launch_bounds__(256) k_discharge(const Params p, uint32_t colu
   __shared__ double lds[4];
   double s = 0.0;
   for (uint32_t y = threadIdx.x; y < p.ny; y += blockDim.x) {
-    const uint32_t n = column + y * p.nx;
+    const uint32_t n = column + y * p.pitch;
     if (!(p.flags[n] & (kWall | kGas))) s += double(p.rho[n]) * double(p.u[n]);
   }
 #pragma unroll
@@ -269,13 +271,13 @@ struct DevBuf {
 
 // Copies the step state (populations and the inlet/outlet publication buffers,
 // plus the readback fields when with_fields) between the handle and buf.
-// Layout of buf: f (9 Np) | pub_rho[0] | pub_rho[1] (N each) | pub_u[0] |
-// pub_u[1] (2N each) | rho (N) | u (2N).
+// Layout of buf (device planes of Nd cells): f (9 Np) | pub_rho[0] |
+// pub_rho[1] (N each) | pub_u[0] | pub_u[1] (2N each) | rho (N) | u (2N).
 size_t state_floats(const flb_handle* h, bool with_fields) {
-  return 9 * h->Np + 6 * h->N + (with_fields ? 3 * h->N : 0);
+  return 9 * h->Np + 6 * h->Nd + (with_fields ? 3 * h->Nd : 0);
 }
 int copy_state(flb_handle* h, float* buf, bool to_handle, bool with_fields) {
-  const size_t N = h->N, Np = h->Np;
+  const size_t N = h->Nd, Np = h->Np;
   struct Part {
     void* dev;
     size_t off, bytes;
@@ -343,13 +345,12 @@ int flb_set_brinkman(flb_handle* h, const float* alpha) {
     return flb::set_error(h, FLB_ERR_UNSUPPORTED, "Brinkman drag needs a monofluid handle");
   if (!h->alpha) {
     void* v = nullptr;
-    ADJ_HIP(h, hipMalloc(&v, h->N * sizeof(float)));
+    ADJ_HIP(h, hipMalloc(&v, h->Nd * sizeof(float)));
     h->owned.push_back(v);
     h->alpha = static_cast<float*>(v);
+    ADJ_HIP(h, hipMemsetAsync(v, 0, h->Nd * sizeof(float), h->stream));
   }
-  ADJ_HIP(h, hipMemcpyAsync(h->alpha, alpha, h->N * sizeof(float), hipMemcpyHostToDevice, h->stream));
-  ADJ_HIP(h, hipStreamSynchronize(h->stream));
-  return FLB_OK;
+  return flb::copy_cells_in(h, h->alpha, alpha, sizeof(float), 1);
 }
 
 int flb_objective(flb_handle* h, const flb_adjoint_spec* sp, double* J) {
@@ -377,7 +378,7 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
                 double* dj_duin, float* phase_ms) {
   int rc = check_adjoint_state(h, sp);
   if (rc) return rc;
-  const size_t N = h->N, Np = h->Np;
+  const size_t N = h->Nd, Np = h->Np;  // device planes
   const uint64_t K = sp->ckpt, T = sp->nsteps;
   const uint64_t nseg = (T + K - 1) / K;
   const size_t ck_floats = state_floats(h, false);
@@ -473,9 +474,8 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
   h->t = 0;
   double acc[2] = {0.0, 0.0};
   ADJ_HIP(h, hipMemcpyAsync(acc, d_acc, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  if (dj_dalpha && galpha)
-    ADJ_HIP(h, hipMemcpyAsync(dj_dalpha, galpha, N * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   ADJ_HIP(h, hipStreamSynchronize(h->stream));
+  if (dj_dalpha && galpha && !rc) rc = flb::copy_cells_out(h, dj_dalpha, galpha, sizeof(float), 1);
   if (phase_ms) {
     float fwd = 0.0f, tot = 0.0f;
     ADJ_HIP(h, hipEventElapsedTime(&fwd, ev[0], ev[1]));
@@ -488,7 +488,7 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
   (void)hipEventDestroy(end);
   if (rc) return rc;
   if (dj_dalpha && !galpha)
-    for (size_t n = 0; n < N; ++n) dj_dalpha[n] = 0.0f;
+    for (size_t n = 0; n < h->N; ++n) dj_dalpha[n] = 0.0f;
   if (J) *J = acc[0];
   if (dj_duin) *dj_duin = acc[1];
   return FLB_OK;

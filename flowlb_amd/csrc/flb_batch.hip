@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(1024) k_batch_step(const Tile* __restrict__ ti
     const Params& p = inst[t.inst];
     const uint32_t x = t.tx * blockDim.x + threadIdx.x;
     if (x < p.nx) {
-      const Cell c{x, t.row, x + t.row * p.nx, true};
+      const Cell c{x, t.row, x + t.row * p.pitch, true};
       stream_collide_cell<S, ODD, false, MOM, false>(p, c);
     }
   }
@@ -260,7 +260,9 @@ int flb_batch_create(const flb_instance* in, uint32_t count, int32_t storage, in
     p = flb::Params{};
     p.nx = s.nx;
     p.ny = s.ny;
+    p.pitch = s.nx;  // instances are packed unpadded
     p.N = N;
+    p.NL = N;
     p.Np = align_up(N, 64);
     p.inv_tau = 1.0f / s.tau;
     p.use_force = s.use_force;

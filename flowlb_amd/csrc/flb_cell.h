@@ -56,26 +56,23 @@ struct Cell {
   bool valid;
 };
 
-// Cell of this thread in row-segment tile L (tiles numbered row by row).
-__device__ __forceinline__ Cell locate_tile(const Params& p, uint32_t L) {
-  const uint32_t y = L / p.tiles_x;
-  const uint32_t x = (L - y * p.tiles_x) * blockDim.x + threadIdx.x;
-  return Cell{x, y, x + y * p.nx, x < p.nx};
-}
-
+// Cell of this thread: block = one row segment of blockDim.x cells.
 __device__ __forceinline__ Cell locate(const Params& p) {
-  return locate_tile(p, xcd_tile(blockIdx.x, p.nblocks));
+  uint32_t tx, y;
+  block_tile(blockIdx.x, p.tiles_x, p.ny, tx, y);
+  const uint32_t x = tx * blockDim.x + threadIdx.x;
+  return Cell{x, y, x + y * p.pitch, x < p.nx};
 }
 
 // Periodic neighbour indices (src/cuda/cellCalculations.cuh:82-109) without
 // integer division: the wrap is a compare at the domain edge.
 __device__ __forceinline__ void neighbours(const Params& p, uint32_t x, uint32_t y, uint32_t* nb) {
-  const uint32_t nx = p.nx;
+  const uint32_t nx = p.nx, pitch = p.pitch;
   const uint32_t xr = x + 1 == nx ? 0u : x + 1;
   const uint32_t xl = x == 0 ? nx - 1 : x - 1;
-  const uint32_t yc = y * nx;
-  const uint32_t yu = (y + 1 == p.ny ? 0u : y + 1) * nx;
-  const uint32_t yd = (y == 0 ? p.ny - 1 : y - 1) * nx;
+  const uint32_t yc = y * pitch;
+  const uint32_t yu = (y + 1 == p.ny ? 0u : y + 1) * pitch;
+  const uint32_t yd = (y == 0 ? p.ny - 1 : y - 1) * pitch;
   nb[0] = x + yc;
   nb[1] = xr + yc;
   nb[2] = xl + yc;
@@ -85,6 +82,25 @@ __device__ __forceinline__ void neighbours(const Params& p, uint32_t x, uint32_t
   nb[6] = xl + yd;
   nb[7] = xr + yd;
   nb[8] = xl + yu;
+}
+
+// The cells before and after (x, y) in the reference's row-major numbering
+// n = x + y * nx, as device indices: the inlet reads rho[n + 1] and the outlet
+// u[n - 1] (lbmKernel2D.cu:336,344), and the last cell of a row is followed by
+// the first cell of the next row.  Where a neighbour does not exist the index
+// is the cell itself and has_* is false.
+struct Linear {
+  uint32_t prev, next;
+  bool has_prev, has_next;
+};
+__device__ __forceinline__ Linear linear_neighbours(const Params& p, uint32_t x, uint32_t y,
+                                                    uint32_t n) {
+  Linear l;
+  l.has_prev = x > 0 || y > 0;
+  l.has_next = x + 1 < p.nx || y + 1 < p.ny;
+  l.prev = x > 0 ? n - 1 : (y > 0 ? n - p.pitch + p.nx - 1 : n);
+  l.next = x + 1 < p.nx ? n + 1 : (y + 1 < p.ny ? n + p.pitch - p.nx + 1 : n);
+  return l;
 }
 
 // Fluid/gas/interface bits of the eight neighbours, issued together so that
@@ -179,13 +195,15 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   const uint64_t N = p.N;
   // Density of cell n is read next step by an inlet at n-1, velocity by an
   // outlet at n+1 (lbmKernel2D.cu:336,344-345); both roles are static.  The
-  // three flags are loaded unconditionally (clamped index, same cache line) so
-  // that they share one memory round trip instead of three branch-guarded ones.
+  // three flags are loaded unconditionally (a missing neighbour reads the cell
+  // itself) so that they share one memory round trip instead of three
+  // branch-guarded ones.
+  const Linear lin = linear_neighbours(p, c.x, c.y, n);
   const uint8_t fl = p.flags[n];
-  const uint8_t fl_prev = p.flags[n > 0 ? n - 1 : n];
-  const uint8_t fl_next = p.flags[n + 1 < N ? n + 1 : n];
-  const bool pub_r = n > 0 && (fl_prev & kInlet);
-  const bool pub_v = n + 1 < N && (fl_next & kOutlet) && !(fl & kInlet);
+  const uint8_t fl_prev = p.flags[lin.prev];
+  const uint8_t fl_next = p.flags[lin.next];
+  const bool pub_r = lin.has_prev && (fl_prev & kInlet);
+  const bool pub_v = lin.has_next && (fl_next & kOutlet) && !(fl & kInlet);
   if (fl & (kGas | kWall)) {
     if (FS) {  // a source cell that is gas this step keeps its last value
       if (pub_r) prho0[n] = prho1[n];
@@ -232,7 +250,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   const uint8_t io = fl & (kInlet | kOutlet);
   R rho, rho1, ux, uy;
   if (io & kInlet) {
-    rho = prho1[n + 1];
+    rho = prho1[lin.next];
     rho1 = rho - R(1);
     ux = u_f[n];
     uy = u_f[N + n];
@@ -244,8 +262,8 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   } else if (fl & kOutlet) {
     rho = rho_f[n];
     rho1 = rho - R(1);
-    ux = pu1[n - 1];
-    uy = pu1[N + n - 1];
+    ux = pu1[lin.prev];
+    uy = pu1[N + lin.prev];
     if (TRACE) {
       p.trace[n] = rho;
       p.trace[N + n] = ux;

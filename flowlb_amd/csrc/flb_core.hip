@@ -1,13 +1,15 @@
 // flb_core.hip -- handle, device memory layout and the C ABI of the FlowLB core.
 //
-// Device memory of one handle (N = nx*ny cells):
-//   f          9 planes x Np elements of the storage type (Np = N rounded up to
-//              a multiple of 64 so every plane starts 256-byte aligned)
-//   flags[2]   N bytes each; free-surface pass 4 writes the second buffer
-//   rho, u     readback fields, N and 2N floats
+// Device memory of one handle.  Every plane stores the nx x ny cells in rows
+// of `pitch` cells (Nd = pitch * ny elements per plane of a cell field):
+//   f          9 planes x Np elements of the storage type (Np = Nd rounded up
+//              to a multiple of 64 so every plane starts 256-byte aligned)
+//   flags[2]   Nd bytes each; free-surface pass 4 writes the second buffer
+//   rho, u     readback fields, 1 and 2 planes
 //   pub_rho[2], pub_u[2]   inlet/outlet neighbour values by step parity
-//   mass, excess, phi      free surface, N floats each
-//   rec        free surface, reconstructed gas populations, 9 x N floats
+//   mass, excess, phi      free surface, 1 plane each
+//   rec        free surface, reconstructed gas populations, 9 planes
+// Host arrays of the ABI hold nx * ny cells per plane, row-major.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -48,9 +50,11 @@ flb::Params params(const flb_handle* h, int bx) {
   flb::Params p{};
   p.nx = h->nx;
   p.ny = h->ny;
+  p.pitch = h->pitch;
   p.tiles_x = uint32_t((h->nx + bx - 1) / bx);
   p.nblocks = p.tiles_x * h->ny;
-  p.N = h->N;
+  p.N = h->Nd;
+  p.NL = h->N;
   p.Np = h->Np;
   p.inv_tau = h->inv_tau;
   const bool force = h->opt.use_force != 0;
@@ -82,7 +86,8 @@ flb::Params params(const flb_handle* h, int bx) {
 template <class T>
 int dev_alloc(flb_handle* h, T** ptr, size_t count) {
   void* v = nullptr;
-  hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&v, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return fail(h, FLB_ERR_MEMORY, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
@@ -90,6 +95,8 @@ int dev_alloc(flb_handle* h, T** ptr, size_t count) {
   }
   h->owned.push_back(v);
   *ptr = static_cast<T*>(v);
+  // Zeroed, so the row padding of pitched planes holds defined values.
+  FLB_HIP(h, hipMemsetAsync(v, 0, bytes, h->stream));
   return FLB_OK;
 }
 
@@ -105,8 +112,26 @@ int download(flb_handle* h, void* dst, const void* src, size_t bytes) {
   return FLB_OK;
 }
 
+// Host planes of nx * ny cells <-> device planes of Nd cells in rows of
+// `pitch`: one strided copy over ny * planes rows (a device plane is exactly
+// ny rows of pitch cells).
+int cells_in(flb_handle* h, void* dev, const void* host, size_t elem, int planes) {
+  FLB_HIP(h, hipMemcpy2DAsync(dev, h->pitch * elem, host, h->nx * elem, h->nx * elem,
+                              size_t(h->ny) * planes, hipMemcpyHostToDevice, h->stream));
+  FLB_HIP(h, hipStreamSynchronize(h->stream));
+  return FLB_OK;
+}
+
+int cells_out(flb_handle* h, void* host, const void* dev, size_t elem, int planes) {
+  FLB_HIP(h, hipMemcpy2DAsync(host, h->nx * elem, dev, h->pitch * elem, h->nx * elem,
+                              size_t(h->ny) * planes, hipMemcpyDeviceToHost, h->stream));
+  FLB_HIP(h, hipStreamSynchronize(h->stream));
+  return FLB_OK;
+}
+
 // Cell fields hold float, or double for FP64 storage (h->field_bytes); the
 // host side of the ABI passes float arrays and these helpers convert.
+// count: device elements.
 int field_alloc(flb_handle* h, float** ptr, size_t count) {
   uint8_t* b = nullptr;
   int rc = dev_alloc(h, &b, count * size_t(h->field_bytes));
@@ -115,32 +140,35 @@ int field_alloc(flb_handle* h, float** ptr, size_t count) {
 }
 
 template <class In>
-int upload_field(flb_handle* h, float* dst, const In* src, size_t count) {
+int upload_field(flb_handle* h, float* dst, const In* src, int planes) {
+  const size_t count = h->N * planes;
   if (h->field_bytes == 8) {
     std::vector<double> t(src, src + count);
-    return upload(h, dst, t.data(), count * sizeof(double));
+    return cells_in(h, dst, t.data(), sizeof(double), planes);
   }
-  if (std::is_same<In, float>::value) return upload(h, dst, src, count * sizeof(float));
+  if (std::is_same<In, float>::value) return cells_in(h, dst, src, sizeof(float), planes);
   std::vector<float> t(src, src + count);
-  return upload(h, dst, t.data(), count * sizeof(float));
+  return cells_in(h, dst, t.data(), sizeof(float), planes);
 }
 
 template <class Out>
-int download_field(flb_handle* h, Out* dst, const float* src, size_t count) {
+int download_field(flb_handle* h, Out* dst, const float* src, int planes) {
+  const size_t count = h->N * planes;
   if (h->field_bytes == 8) {
-    if (std::is_same<Out, double>::value) return download(h, dst, src, count * sizeof(double));
+    if (std::is_same<Out, double>::value) return cells_out(h, dst, src, sizeof(double), planes);
     std::vector<double> t(count);
-    int rc = download(h, t.data(), src, count * sizeof(double));
+    int rc = cells_out(h, t.data(), src, sizeof(double), planes);
     for (size_t i = 0; i < count && !rc; ++i) dst[i] = Out(t[i]);
     return rc;
   }
-  if (std::is_same<Out, float>::value) return download(h, dst, src, count * sizeof(float));
+  if (std::is_same<Out, float>::value) return cells_out(h, dst, src, sizeof(float), planes);
   std::vector<float> t(count);
-  int rc = download(h, t.data(), src, count * sizeof(float));
+  int rc = cells_out(h, t.data(), src, sizeof(float), planes);
   for (size_t i = 0; i < count && !rc; ++i) dst[i] = Out(t[i]);
   return rc;
 }
 
+// count: device elements.
 int zero_field(flb_handle* h, float* dst, size_t count) {
   FLB_HIP(h, hipMemsetAsync(dst, 0, count * size_t(h->field_bytes), h->stream));
   return FLB_OK;
@@ -205,8 +233,21 @@ int validate(const flb_options* o, uint32_t nx, uint32_t ny, std::string& why) {
   if (o->block_x != 0 && (o->block_x < 64 || o->block_x > 1024 || o->block_x % 64))
     return why = "block_x must be a multiple of 64 in [64, 1024]", 1;
   if (nx < 2 || ny < 2) return why = "grid must be at least 2 x 2", 1;
-  if (uint64_t(nx) * ny > 0xffffffffull / 2) return why = "grid too large for 32-bit cell indices", 1;
+  if (o->row_pitch != 0 && (o->row_pitch < 0 || uint32_t(o->row_pitch) < nx))
+    return why = "row_pitch must be 0 or at least nx", 1;
+  if ((uint64_t(nx) + 128) * ny > 0xffffffffull / 2 ||
+      uint64_t(o->row_pitch) * ny > 0xffffffffull / 2)
+    return why = "grid too large for 32-bit cell indices", 1;
   return 0;
+}
+
+// Row stride of the device planes: nx rounded up to 64 cells, so each
+// wavefront row segment starts 256-byte aligned.  (A further 64 cells of
+// padding for rows of a multiple of 4 KiB measured the same on the
+// 8192 x 2048 dam break: stream-collide 170 vs 167 us.)
+uint32_t row_pitch(const flb_options* o, uint32_t nx) {
+  if (o->row_pitch > 0) return uint32_t(o->row_pitch);
+  return (nx + 63u) / 64u * 64u;
 }
 
 }  // namespace
@@ -247,6 +288,13 @@ int flb_get_shape(const flb_handle* h, uint32_t* nx, uint32_t* ny, int32_t* prob
   return FLB_OK;
 }
 
+int flb_get_layout(const flb_handle* h, uint32_t* pitch, uint64_t* plane) {
+  if (!h) return FLB_ERR_ARGUMENT;
+  if (pitch) *pitch = h->pitch;
+  if (plane) *plane = h->Nd;
+  return FLB_OK;
+}
+
 void flb_destroy(flb_handle* h) {
   if (!h) return;
   if (h->stream) {
@@ -276,7 +324,9 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   h->nx = nx;
   h->ny = ny;
   h->N = N;
-  h->Np = round_up(N, 64);
+  h->pitch = row_pitch(opt, nx);
+  h->Nd = uint64_t(h->pitch) * ny;
+  h->Np = round_up(h->Nd, 64);
   h->elem_bytes = opt->storage == FLB_STORAGE_FP32 ? 4 : (opt->storage == FLB_STORAGE_FP64 ? 8 : 2);
   h->field_bytes = opt->storage == FLB_STORAGE_FP64 ? 8 : 4;
   h->inv_tau = 1.0f / opt->tau;
@@ -299,22 +349,23 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   if ((rc = dev_alloc(h, &fbytes, 9 * h->Np * h->elem_bytes))) return bail(rc);
   h->f = fbytes;
   const bool fs = opt->problem == FLB_FREE_SURFACE;
-  if ((rc = dev_alloc(h, &h->flags[0], N))) return bail(rc);
-  if ((rc = dev_alloc(h, &h->flags[1], fs ? N : 1))) return bail(rc);
-  if ((rc = field_alloc(h, &h->rho, N))) return bail(rc);
-  if ((rc = field_alloc(h, &h->u, 2 * N))) return bail(rc);
+  const uint64_t Nd = h->Nd;
+  if ((rc = dev_alloc(h, &h->flags[0], Nd))) return bail(rc);
+  if ((rc = dev_alloc(h, &h->flags[1], fs ? Nd : 1))) return bail(rc);
+  if ((rc = field_alloc(h, &h->rho, Nd))) return bail(rc);
+  if ((rc = field_alloc(h, &h->u, 2 * Nd))) return bail(rc);
   for (int k = 0; k < 2; ++k) {
-    if ((rc = field_alloc(h, &h->pub_rho[k], N))) return bail(rc);
-    if ((rc = field_alloc(h, &h->pub_u[k], 2 * N))) return bail(rc);
+    if ((rc = field_alloc(h, &h->pub_rho[k], Nd))) return bail(rc);
+    if ((rc = field_alloc(h, &h->pub_u[k], 2 * Nd))) return bail(rc);
   }
   if (fs) {
-    if ((rc = field_alloc(h, &h->mass, N))) return bail(rc);
-    if ((rc = field_alloc(h, &h->excess, N))) return bail(rc);
-    if ((rc = field_alloc(h, &h->phi, N))) return bail(rc);
-    if ((rc = field_alloc(h, &h->rec, 9 * N))) return bail(rc);
+    if ((rc = field_alloc(h, &h->mass, Nd))) return bail(rc);
+    if ((rc = field_alloc(h, &h->excess, Nd))) return bail(rc);
+    if ((rc = field_alloc(h, &h->phi, Nd))) return bail(rc);
+    if ((rc = field_alloc(h, &h->rec, 9 * Nd))) return bail(rc);
   }
-  if ((rc = dev_alloc(h, &h->rho_init, N))) return bail(rc);
-  if ((rc = dev_alloc(h, &h->u_init, 2 * N))) return bail(rc);
+  if ((rc = dev_alloc(h, &h->rho_init, Nd))) return bail(rc);
+  if ((rc = dev_alloc(h, &h->u_init, 2 * Nd))) return bail(rc);
   const flb::Params pu = params(h, kUtilBlock);
   h->partial_len = pu.nblocks;
   if ((rc = dev_alloc(h, &h->partial, h->partial_len))) return bail(rc);
@@ -324,20 +375,15 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   std::vector<float> hr(N, 1.0f), hu(2 * N, 0.0f);
   if (rho0) std::memcpy(hr.data(), rho0, N * sizeof(float));
   if (u0) std::memcpy(hu.data(), u0, 2 * N * sizeof(float));
-  if ((rc = upload(h, h->flags[0], flags, N))) return bail(rc);
-  if (fs && (rc = upload(h, h->flags[1], flags, N))) return bail(rc);
-  if ((rc = upload_field(h, h->rho, hr.data(), N))) return bail(rc);
-  if ((rc = upload_field(h, h->u, hu.data(), 2 * N))) return bail(rc);
-  if ((rc = upload(h, h->rho_init, hr.data(), N * sizeof(float)))) return bail(rc);
-  if ((rc = upload(h, h->u_init, hu.data(), 2 * N * sizeof(float)))) return bail(rc);
+  if ((rc = cells_in(h, h->flags[0], flags, 1, 1))) return bail(rc);
+  if (fs && (rc = cells_in(h, h->flags[1], flags, 1, 1))) return bail(rc);
+  if ((rc = upload_field(h, h->rho, hr.data(), 1))) return bail(rc);
+  if ((rc = upload_field(h, h->u, hu.data(), 2))) return bail(rc);
+  if ((rc = cells_in(h, h->rho_init, hr.data(), sizeof(float), 1))) return bail(rc);
+  if ((rc = cells_in(h, h->u_init, hu.data(), sizeof(float), 2))) return bail(rc);
   for (int k = 0; k < 2; ++k) {
-    if ((rc = upload_field(h, h->pub_rho[k], hr.data(), N))) return bail(rc);
-    if ((rc = upload_field(h, h->pub_u[k], hu.data(), 2 * N))) return bail(rc);
-  }
-  if (fs) {
-    if ((rc = zero_field(h, h->mass, N)) || (rc = zero_field(h, h->excess, N)) ||
-        (rc = zero_field(h, h->phi, N)) || (rc = zero_field(h, h->rec, 9 * N)))
-      return bail(rc);
+    if ((rc = upload_field(h, h->pub_rho[k], hr.data(), 1))) return bail(rc);
+    if ((rc = upload_field(h, h->pub_u[k], hu.data(), 2))) return bail(rc);
   }
   // Populations: shifted feq, converted to the storage format on the device
   // (from double for FP64 storage, from float otherwise).
@@ -376,13 +422,12 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
 int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const float* excess) {
   if (!h || !mass || !phi) return h ? fail(h, FLB_ERR_ARGUMENT, "mass and phi required") : FLB_ERR_ARGUMENT;
   if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "not a free-surface handle");
-  const size_t n = h->N;
   int rc;
-  if ((rc = upload_field(h, h->mass, mass, n))) return rc;
-  if ((rc = upload_field(h, h->phi, phi, n))) return rc;
-  if ((rc = zero_field(h, h->rec, n))) return rc;  // stream-collide's mass residuals
-  if (excess) return upload_field(h, h->excess, excess, n);
-  return zero_field(h, h->excess, n);
+  if ((rc = upload_field(h, h->mass, mass, 1))) return rc;
+  if ((rc = upload_field(h, h->phi, phi, 1))) return rc;
+  if ((rc = zero_field(h, h->rec, h->Nd))) return rc;  // stream-collide's mass residuals
+  if (excess) return upload_field(h, h->excess, excess, 1);
+  return zero_field(h, h->excess, h->Nd);
 }
 
 // One lattice step on the handle's stream.  ev (optional) receives per-kernel
@@ -431,6 +476,12 @@ Params make_params(const flb_handle* h, int bx) { return params(h, bx); }
 int handle_block_x(const flb_handle* h) { return block_x(h); }
 int enqueue_step_ex(flb_handle* h, bool moments, hipEvent_t* ev) {
   return enqueue_step(h, moments, ev);
+}
+int copy_cells_in(flb_handle* h, void* dev, const void* host, size_t elem, int planes) {
+  return cells_in(h, dev, host, elem, planes);
+}
+int copy_cells_out(flb_handle* h, void* host, const void* dev, size_t elem, int planes) {
+  return cells_out(h, host, dev, elem, planes);
 }
 }  // namespace flb
 
@@ -499,14 +550,13 @@ namespace {
 template <class T>
 int read_fields(flb_handle* h, T* u, T* rho, uint8_t* flags, T* phi) {
   if (!h) return FLB_ERR_ARGUMENT;
-  const size_t N = h->N;
   int rc;
-  if (u && (rc = download_field(h, u, h->u, 2 * N))) return rc;
-  if (rho && (rc = download_field(h, rho, h->rho, N))) return rc;
-  if (flags && (rc = download(h, flags, h->flags[h->cur], N))) return rc;
+  if (u && (rc = download_field(h, u, h->u, 2))) return rc;
+  if (rho && (rc = download_field(h, rho, h->rho, 1))) return rc;
+  if (flags && (rc = cells_out(h, flags, h->flags[h->cur], 1, 1))) return rc;
   if (phi) {
     if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "phi needs free surface");
-    if ((rc = download_field(h, phi, h->phi, N))) return rc;
+    if ((rc = download_field(h, phi, h->phi, 1))) return rc;
   }
   return FLB_OK;
 }
@@ -516,8 +566,8 @@ int read_fs_fields(flb_handle* h, T* mass, T* excess) {
   if (!h) return FLB_ERR_ARGUMENT;
   if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "not a free-surface handle");
   int rc;
-  if (mass && (rc = download_field(h, mass, h->mass, h->N))) return rc;
-  if (excess && (rc = download_field(h, excess, h->excess, h->N))) return rc;
+  if (mass && (rc = download_field(h, mass, h->mass, 1))) return rc;
+  if (excess && (rc = download_field(h, excess, h->excess, 1))) return rc;
   return FLB_OK;
 }
 

@@ -95,11 +95,17 @@ struct StoreBF16S {
 constexpr int kFsTileW = 64, kFsTileH = 8;
 
 // ---------------------------------------------------------------- params
+// Device cell fields and population planes store a row of nx cells every
+// `pitch` cells (pitch >= nx): cell (x, y) is element x + y * pitch of a plane.
+// The default pitch rounds nx up to 64, so every wavefront row segment starts
+// 256-byte aligned (FP32).
 struct Params {
   uint32_t nx, ny;
+  uint32_t pitch;        // row stride of the device planes (cells)
   uint32_t tiles_x;      // blocks per row
   uint32_t nblocks;      // total blocks of a full-grid launch
-  uint64_t N;            // cells
+  uint64_t N;            // plane size of the device cell fields (pitch * ny)
+  uint64_t NL;           // cells of the domain (nx * ny): host arrays, staging
   uint64_t Np;           // plane stride of the population array (elements)
   float inv_tau, fx, fy;
   int32_t use_force, collision, smagorinsky, gas_check_corrected, mass_mode;
@@ -109,7 +115,7 @@ struct Params {
   // Cell fields.  They hold elements of the storage's arithmetic type R
   // (double for StoreF64); kernels access them through fld<R>().
   float* rho;            // readback density (N)
-  float* u;              // readback velocity (2N)
+  float* u;              // readback velocity (2 planes of N)
   float* pub_rho[2];     // densities published for inlet cells, by parity
   float* pub_u[2];       // velocities published for outlet cells, by parity
   float* mass;
@@ -126,13 +132,36 @@ __device__ __forceinline__ R* fld(float* p) {
   return reinterpret_cast<R*>(p);
 }
 
-// Block -> tile mapping.  Workgroups are dispatched round-robin over the 8
-// XCDs (block b to XCD b mod 8), each with its own L2.  Tiles are numbered row
-// by row and taken in that order: with a row of a multiple of 8 tiles the tiles
-// above and below a tile run on the same XCD, so the row-shifted loads meet in
-// its L2, and every XCD gets an equal share of every band of rows.  (Giving
-// each XCD a contiguous band of rows instead left the XCDs holding the gas
-// rows of a free-surface domain idle while the others worked on the water.)
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t) { return b; }
+// Block -> tile mapping.  Workgroups are dispatched round-robin: block b goes
+// to XCD b mod 8, and within an XCD consecutive blocks go to its shader
+// engines in turn, so block b lands on one of 32 fixed dispatch slots by
+// b mod 32.  In row-major order with a multiple of 32 tiles per row every slot
+// owns fixed column strips: on the 8192 x 2048 dam break (rows wet left of
+// x = 3600, 32 tiles per row) half of the shader engines never saw a wet tile
+// and stream-collide took 177 us.  Blocks therefore walk bands of kBandRows
+// rows column by column (block b -> row band * kBandRows + b mod h of its
+// band of h rows): the slot follows the row, every slot gets every column,
+// and stream-collide takes 118 us (single phase at 8192 x 2048: 286 -> 254
+// us).  Rotating each row by y tiles was as fast at 32 tiles per row but not
+// at 33.  A whole band (32 rows) is in flight at once, as 32 rows were in
+// row-major order.
+constexpr uint32_t kTileOrder = 2;  // 0 row-major, 1 rows rotated by y, 2 bands
+constexpr uint32_t kBandRows = 32;
+__device__ __forceinline__ void block_tile(uint32_t b, uint32_t tiles_x, uint32_t rows,
+                                           uint32_t& tx, uint32_t& y) {
+  if (kTileOrder == 2) {
+    const uint32_t per_band = kBandRows * tiles_x;
+    const uint32_t band = b / per_band;
+    const uint32_t r = b - band * per_band;
+    const uint32_t y0 = band * kBandRows;
+    const uint32_t h = rows - y0 < kBandRows ? rows - y0 : kBandRows;
+    tx = r / h;
+    y = y0 + (r - tx * h);
+    return;
+  }
+  y = b / tiles_x;
+  tx = b - y * tiles_x;
+  if (kTileOrder == 1) tx = (tx + y % tiles_x) % tiles_x;
+}
 
 }  // namespace flb

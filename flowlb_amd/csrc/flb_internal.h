@@ -32,7 +32,10 @@ HistFn get_flag_hist();
 struct flb_handle {
   flb_options opt{};
   uint32_t nx = 0, ny = 0;
-  uint64_t N = 0, Np = 0;
+  uint32_t pitch = 0;  // row stride of the device planes (cells)
+  uint64_t N = 0;      // cells, nx * ny (host arrays)
+  uint64_t Nd = 0;     // device plane of a cell field, pitch * ny
+  uint64_t Np = 0;     // population plane stride, Nd rounded up to 64
   int elem_bytes = 4;   // bytes per stored population
   int field_bytes = 4;  // bytes per element of the cell fields (8 for FP64)
   hipStream_t stream = nullptr;
@@ -65,6 +68,11 @@ int set_error(flb_handle* h, int code, const std::string& msg);
 Params make_params(const flb_handle* h, int bx);
 int handle_block_x(const flb_handle* h);
 int enqueue_step_ex(flb_handle* h, bool moments, hipEvent_t* ev);
+// Copies `planes` planes of cells of `elem` bytes between a host array
+// (planes of nx * ny, row-major) and a pitched device array (planes of Nd),
+// on the handle's stream, and waits for the copy.
+int copy_cells_in(flb_handle* h, void* dev, const void* host, size_t elem, int planes);
+int copy_cells_out(flb_handle* h, void* host, const void* dev, size_t elem, int planes);
 // Shifted initial populations feq(rho0, u0) in the odd-step slot pattern.
 void host_initial_populations(uint32_t nx, uint32_t ny, const float* rho0, const float* u0,
                               std::vector<float>& fs);

@@ -349,14 +349,14 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   // f0s is padded to two bytes per thread so that every lane writes both of
   // its halo bytes unconditionally (lanes past the tile write the padding).
   __shared__ uint8_t f0s[2 * kUpdateThreads], f1s[kLdsH * kLdsW], f2s[kLdsH * kLdsW];
-  const int nx = int(p.nx), ny = int(p.ny);
-  const uint32_t b = xcd_tile(blockIdx.x, p.nblocks);
-  const uint32_t ty = b / p.tiles_x, tx = b - ty * p.tiles_x;
+  const int nx = int(p.nx), ny = int(p.ny), pitch = int(p.pitch);
+  uint32_t tx, ty;
+  block_tile(blockIdx.x, p.tiles_x, (p.ny + kTileH - 1) / kTileH, tx, ty);
   const int x0 = int(tx) * kTileW, y0 = int(ty) * kTileH;
   const int tid = int(threadIdx.x);
   const int x = x0 + (tid & (kTileW - 1)), y = y0 + tid / kTileW;
   const bool valid = x < nx && y < ny;
-  const uint32_t n = valid ? uint32_t(x + y * nx) : 0u;
+  const uint32_t n = valid ? uint32_t(x + y * pitch) : 0u;
   // All global loads of the tile are issued before any is used: the two halo
   // bytes of this thread, its own flag and mass.  The density of a non-gas
   // cell depends only on the own flag, so its load follows at once and
@@ -373,14 +373,36 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   const uint8_t own = p.flags[n];
   R m = mass[n];
   const uint8_t h0 =
-      p.flags[wrap<SMALL>(x0 + c0 - kHalo, nx) + wrap<SMALL>(y0 + r0 - kHalo, ny) * nx];
+      p.flags[wrap<SMALL>(x0 + c0 - kHalo, nx) + wrap<SMALL>(y0 + r0 - kHalo, ny) * pitch];
   const uint8_t h1 =
-      p.flags[wrap<SMALL>(x0 + c1 - kHalo, nx) + wrap<SMALL>(y0 + r1 - kHalo, ny) * nx];
+      p.flags[wrap<SMALL>(x0 + c1 - kHalo, nx) + wrap<SMALL>(y0 + r1 - kHalo, ny) * pitch];
   const bool dense = valid && !(own & (kGas | kWall | kMovingWall | kInlet));
   const R rho_pre = fld<R>(p.rho)[dense ? n : 0u];
   f0s[i0] = h0;
   f0s[i1] = h1;
+  // A flag changes in passes 2-4 only within three cells of a fluid or
+  // interface cell, and only a gas cell holding mass needs more than its flag
+  // copied.  Tiles without either (most of a dam-break domain) skip the passes.
+  constexpr uint8_t kWet = kFluid | kInterface | kGasInterface;
+  const bool own_plain = !valid || (own & (kWall | kMovingWall | kInlet)) ||
+                         ((own & (kFGI | kGasInterface)) == kGas && m == R(0));
+  const bool busy = (h0 & kWet) || (has1 && (h1 & kWet)) || !own_plain;
+  __shared__ uint64_t busy_waves;  // one byte per wavefront
+  static_assert(kUpdateThreads / 64 <= 8, "one byte per wavefront");
+  const bool wave_busy = __any(busy);
+  if ((tid & 63) == 0) reinterpret_cast<uint8_t*>(&busy_waves)[tid >> 6] = wave_busy;
   __syncthreads();
+  if (!busy_waves) {
+    if (valid) {
+      p.flags_next[n] = own;
+      if (!(own & (kWall | kMovingWall | kInlet)) && (own & (kFGI | kGasInterface)) == kGas &&
+          signbit(m)) {
+        mass[n] = R(0);
+        fld<R>(p.excess)[n] = R(0);
+      }
+    }
+    return;
+  }
   for (int i = tid; i < (kLdsH - 2) * (kLdsW - 2); i += kUpdateThreads) {  // pass 2
     const int r = 1 + i / (kLdsW - 2), j = r * kLdsW + 1 + i % (kLdsW - 2);
     const uint8_t fl = f0s[j];
@@ -461,24 +483,27 @@ __global__ void __launch_bounds__(256) k_pop_mass(const Params p, double* partia
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// Full populations f = (f - w) + w, in the output type O (float or double).
+// Full populations f = (f - w) + w, in the output type O (float or double),
+// to a staging array of 9 planes of nx * ny cells in row-major order.
 template <class S, class O>
 __global__ void k_pop_out(const Params p, O* out) {
   using R = typename S::R;
   const Cell c = locate(p);
   if (!c.valid) return;
   const typename S::T* f = static_cast<const typename S::T*>(p.f);
-  for (int q = 0; q < 9; q++) out[q * p.N + c.n] = O(S::load(f + q * p.Np, c.n) + wgt_r<R>(q));
+  const uint64_t m = c.x + uint64_t(c.y) * p.nx;
+  for (int q = 0; q < 9; q++) out[q * p.NL + m] = O(S::load(f + q * p.Np, c.n) + wgt_r<R>(q));
 }
 
-// in: shifted populations (f - w) of type I, plane stride N.
+// in: shifted populations (f - w) of type I, 9 planes of nx * ny cells.
 template <class S, class I>
 __global__ void k_pop_in(const Params p, const I* in) {
   using R = typename S::R;
   const Cell c = locate(p);
   if (!c.valid) return;
   typename S::T* f = static_cast<typename S::T*>(p.f);
-  for (int q = 0; q < 9; q++) S::store(f + q * p.Np, c.n, R(in[q * p.N + c.n]));
+  const uint64_t m = c.x + uint64_t(c.y) * p.nx;
+  for (int q = 0; q < 9; q++) S::store(f + q * p.Np, c.n, R(in[q * p.NL + m]));
 }
 
 // Flag histogram, grid-stride over all cells.  The lanes of a wavefront that
@@ -491,8 +516,9 @@ __global__ void __launch_bounds__(256) k_flag_hist(const Params p, unsigned long
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; n < p.N; n += stride) {
-    const int v = p.flags[n];
+  for (uint64_t m = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; m < p.NL; m += stride) {
+    const uint64_t y = m / p.nx;
+    const int v = p.flags[m + y * (p.pitch - p.nx)];  // skip the row padding
     uint64_t todo = __ballot(1);
     while (todo) {
       const int leader = __ffsll((unsigned long long)todo) - 1;

@@ -91,6 +91,8 @@ typedef struct flb_options {
   int32_t mass_mode;           /* FLB_MASS_* */
   int32_t block_x;             /* threads per block along x (64..1024, multiple of 64); 0 = 256 */
   int32_t device;              /* HIP device ordinal */
+  int32_t row_pitch;           /* cells per row of the device arrays (>= nx); 0 = nx
+                                  rounded up to 64 (see flb_get_layout) */
 } flb_options;
 
 typedef struct flb_handle flb_handle;
@@ -144,9 +146,11 @@ int flb_write_populations(flb_handle* h, const float* f);
 /* Device pointers of the fields, for a HIP-GL interop consumer
    (the reference maps u onto a GL buffer, src/cuda/runCuda.cu:137-147).
    Any output pointer may be NULL.  u, rho and phi hold doubles for FP64
-   storage, floats otherwise.  They stay valid until flb_destroy; the flag
-   buffer of a free-surface handle alternates every step, so query it again
-   after stepping. */
+   storage, floats otherwise.  Device fields are pitched: cell (x, y) is
+   element x + y * pitch of a plane of `plane` elements, and u holds the ux
+   plane followed by the uy plane (flb_get_layout).  They stay valid until
+   flb_destroy; the flag buffer of a free-surface handle alternates every
+   step, so query it again after stepping. */
 int flb_device_fields(flb_handle* h, void** d_u, void** d_rho, void** d_flags, void** d_phi);
 
 /* Total fluid mass in FP64: stored mass plus the excess-mass shares that the
@@ -161,6 +165,11 @@ uint64_t flb_time(const flb_handle* h);
 /* Grid size and problem type (FLB_MONOFLUID / FLB_FREE_SURFACE) of a handle;
    any output pointer may be NULL. */
 int flb_get_shape(const flb_handle* h, uint32_t* nx, uint32_t* ny, int32_t* problem);
+
+/* Layout of the device fields of flb_device_fields: row stride `pitch`
+   (cells, >= nx) and plane size `plane` (= pitch * ny elements).  No
+   reference counterpart (the reference's device fields are unpadded). */
+int flb_get_layout(const flb_handle* h, uint32_t* pitch, uint64_t* plane);
 
 /* Enqueue nsteps steps between two HIP events on the handle's stream and
    return the elapsed device time in ms.  With per_kernel_ms non-NULL (array of

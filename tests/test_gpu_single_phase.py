@@ -181,3 +181,51 @@ def test_readback_helpers():
             assert h[v] == cnt
         ptrs = s.device_fields()
         assert ptrs[0] and ptrs[1] and ptrs[2]
+
+
+@pytest.mark.parametrize("pitch", [0, 320, 259])
+def test_row_pitch_does_not_change_results(pitch):
+    """Device planes padded to other row strides (flb_options.row_pitch) give
+    bit-identical populations and fields: the layout is storage only.  The
+    256 x 64 culvert has inlet and outlet columns, whose linear neighbours
+    n +- 1 are read through the pitch."""
+    case = cases.scaled(SP, 2)
+    with case.solver() as a, case.solver(row_pitch=pitch) as b:
+        assert a.layout == (256, 256 * 64)          # nx is a multiple of 64
+        assert b.layout == (pitch or 256, (pitch or 256) * 64)
+        a.step(40)
+        b.step(40)
+        ga, gb = a.read(), b.read()
+        np.testing.assert_array_equal(a.populations(), b.populations())
+        np.testing.assert_array_equal(ga["u"], gb["u"])
+        np.testing.assert_array_equal(ga["rho"], gb["rho"])
+        np.testing.assert_array_equal(ga["flags"], gb["flags"])
+        np.testing.assert_array_equal(a.flag_histogram(), b.flag_histogram())
+        assert a.mass() == pytest.approx(b.mass(), rel=1e-14)
+
+
+def test_row_pitch_rejected_below_nx():
+    case = cases.scaled(SP, 8)
+    with pytest.raises(core.FlbError):
+        case.solver(row_pitch=case.nx - 1)
+
+
+def test_inlet_on_last_column_reads_next_row(orc):
+    """An inlet in the last column reads rho[n + 1], which in the reference's
+    row-major numbering is the first cell of the next row, and an outlet in
+    the first column reads u[n - 1], the last cell of the previous row
+    (lbmKernel2D.cu:336,344).  With padded device rows those are not the
+    adjacent elements; the populations must still match the oracle."""
+    nx, ny = 64, 16
+    flags, rho0, u0 = core.build_culvert(nx, ny, False, (core.OUTLET, core.INLET, core.WALL, core.WALL),
+                                         (24, 32), 6, u_inlet=-0.03, rho_noise=1e-4, seed=3)
+    opts = core.default_options(tau=0.6, row_pitch=nx + 64)
+    with core.Solver(flags, nx, ny, opts, rho0=rho0, u0=u0) as s:
+        s.step(9)
+        f = s.populations()
+        g = s.read()
+    o = orc.Oracle(flags, nx, ny, 0.6, rho0=rho0.astype(np.float64), u0=u0.astype(np.float64))
+    o.step(9)
+    r = o.get()
+    np.testing.assert_allclose(f, r["f"], rtol=0, atol=2e-6)
+    np.testing.assert_allclose(g["u"], r["u"], rtol=0, atol=2e-6)
