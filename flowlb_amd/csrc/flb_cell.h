@@ -64,6 +64,23 @@ __device__ __forceinline__ Cell locate(const Params& p) {
   return Cell{x, y, x + y * p.pitch, x < p.nx};
 }
 
+// Free surface: does the 64-cell row chunk of this wavefront have work for the
+// pass whose bits start at `base` (kWorkSC, kWorkFS1)?  In conservative mode
+// pass 1 has none in chunks of gas and interior fluid, stream-collide none in
+// chunks of gas and walls.  The chunk is uniform over the wavefront (a block
+// is one row segment that starts at a multiple of 64), so its tile word is a
+// scalar load.
+__device__ __forceinline__ bool chunk_has(const Params& p, const Cell& c, int base,
+                                          bool with_pub) {
+  const uint32_t tx = uint32_t(__builtin_amdgcn_readfirstlane(c.x)) / kFsTileW;
+  const uint32_t y = uint32_t(__builtin_amdgcn_readfirstlane(c.y));
+  if (tx >= p.work_x) return false;  // the wavefront lies past the row's end
+  const uint32_t i = (y / kFsTileH) * p.work_x + tx;
+  uint32_t w = p.tile_work[i];
+  if (with_pub) w |= p.tile_pub[i];
+  return (w >> (base + int(y % kFsTileH))) & 1u;
+}
+
 // Periodic neighbour indices (src/cuda/cellCalculations.cuh:82-109) without
 // integer division: the wrap is a compare at the domain edge.
 __device__ __forceinline__ void neighbours(const Params& p, uint32_t x, uint32_t y, uint32_t* nb) {

@@ -80,6 +80,9 @@ flb::Params params(const flb_handle* h, int bx) {
   p.rec = h->rec;
   p.alpha = h->alpha;
   p.trace = nullptr;
+  p.tile_work = h->tile_work;
+  p.tile_pub = h->tile_pub;
+  p.work_x = h->work_x;
   return p;
 }
 
@@ -363,6 +366,23 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
     if ((rc = field_alloc(h, &h->excess, Nd))) return bail(rc);
     if ((rc = field_alloc(h, &h->phi, Nd))) return bail(rc);
     if ((rc = field_alloc(h, &h->rec, 9 * Nd))) return bail(rc);
+    // Tile work words: every chunk has work before the first update kernel
+    // has written them.
+    h->work_x = (nx + flb::kFsTileW - 1) / flb::kFsTileW;
+    const size_t tiles = size_t(h->work_x) * ((ny + flb::kFsTileH - 1) / flb::kFsTileH);
+    if ((rc = dev_alloc(h, &h->tile_work, tiles))) return bail(rc);
+    if ((rc = dev_alloc(h, &h->tile_pub, tiles))) return bail(rc);
+    std::vector<uint32_t> pub(tiles, 0u), all(tiles, 0xffffffffu);
+    for (uint64_t n = 0; n < N; ++n) {
+      const bool pub_r = n > 0 && (flags[n - 1] & FLB_INLET);
+      const bool pub_v = n + 1 < N && (flags[n + 1] & FLB_OUTLET) && !(flags[n] & FLB_INLET);
+      const uint64_t x = n % nx, y = n / nx;
+      if (pub_r || pub_v)
+        pub[(y / flb::kFsTileH) * h->work_x + x / flb::kFsTileW] |=
+            1u << (flb::kWorkSC + int(y % flb::kFsTileH));
+    }
+    if ((rc = upload(h, h->tile_pub, pub.data(), tiles * sizeof(uint32_t)))) return bail(rc);
+    if ((rc = upload(h, h->tile_work, all.data(), tiles * sizeof(uint32_t)))) return bail(rc);
   }
   if ((rc = dev_alloc(h, &h->rho_init, Nd))) return bail(rc);
   if ((rc = dev_alloc(h, &h->u_init, 2 * Nd))) return bail(rc);
@@ -425,6 +445,11 @@ int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const 
   int rc;
   if ((rc = upload_field(h, h->mass, mass, 1))) return rc;
   if ((rc = upload_field(h, h->phi, phi, 1))) return rc;
+  {  // new masses: every tile is evaluated again in the next update
+    const size_t tiles = size_t(h->work_x) * ((h->ny + flb::kFsTileH - 1) / flb::kFsTileH);
+    std::vector<uint32_t> all(tiles, 0xffffffffu);
+    if ((rc = upload(h, h->tile_work, all.data(), tiles * sizeof(uint32_t)))) return rc;
+  }
   if ((rc = zero_field(h, h->rec, h->Nd))) return rc;  // stream-collide's mass residuals
   if (excess) return upload_field(h, h->excess, excess, 1);
   return zero_field(h, h->excess, h->Nd);

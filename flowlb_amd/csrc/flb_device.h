@@ -93,6 +93,7 @@ struct StoreBF16S {
 // Tile of the fused free-surface passes 2-4: one block of kFsTileW x kFsTileH
 // threads per tile, one wavefront per tile row.
 constexpr int kFsTileW = 64, kFsTileH = 8;
+static_assert(kFsTileH <= 8, "one byte of row bits per pass in a tile_work word");
 
 // ---------------------------------------------------------------- params
 // Device cell fields and population planes store a row of nx cells every
@@ -124,7 +125,19 @@ struct Params {
   float* rec;            // reconstructed gas populations, 9 planes of N
   const float* alpha;    // Brinkman solid fraction or nullptr
   float* trace;          // adjoint trace of one step: 3 planes of N (or nullptr)
+  // Free surface: one word per kFsTileW x kFsTileH tile (work_x per tile row)
+  // telling which of its 64-cell row chunks have work next step: bit r for
+  // stream-collide in row r of the tile, bit 8 + r for pass 1.  Written by the
+  // update kernel; tile_pub holds the stream-collide bits of the chunks with a
+  // cell that publishes for an inlet/outlet neighbour (static).
+  uint32_t* tile_work;
+  const uint32_t* tile_pub;
+  uint32_t work_x;
 };
+
+constexpr int kWorkSC = 0, kWorkFS1 = 8;  // bit offsets in a tile_work word
+constexpr uint32_t kWorkQuiet = 1u << 16;  // the update found the tile quiet
+constexpr uint32_t kWorkSCMask = 0xffu << kWorkSC;
 
 // A cell field viewed in the arithmetic type R.
 template <class R>

@@ -50,6 +50,9 @@ struct flb_handle {
   float* excess = nullptr;
   float* phi = nullptr;
   float* rec = nullptr;
+  uint32_t* tile_work = nullptr;  // free surface: per-tile work words (Params::tile_work)
+  uint32_t* tile_pub = nullptr;   // free surface: chunks holding an inlet/outlet publisher
+  uint32_t work_x = 0;
   float* alpha = nullptr;     // Brinkman solid fraction (allocated on first use)
   float* rho_init = nullptr;  // rho0 and u0 as given to flb_create
   float* u_init = nullptr;

@@ -28,6 +28,7 @@ namespace flb {
 template <class S, int ODD, bool FS, bool MOM, bool TRACE = false>
 __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
   const Cell c = locate(p);
+  if (FS && !chunk_has(p, c, kWorkSC, true)) return;
   if (c.valid) stream_collide_cell<S, ODD, FS, MOM, TRACE>(p, c);
 }
 
@@ -156,6 +157,7 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
 template <class S, int ODD>
 __global__ void __launch_bounds__(1024) k_fs1(const Params p) {
   const Cell c = locate(p);
+  if (!chunk_has(p, c, kWorkFS1, false)) return;
   if (c.valid) fs1_cell<S, ODD>(p, c);
 }
 
@@ -219,15 +221,16 @@ __device__ __forceinline__ int lds_offset(int i) {
 // f0 its flag after stream-collide, f2s[j] its flag after pass 3 (neighbours
 // at lds_offset), m its mass.
 template <class S, int ODD>
-__device__ __forceinline__ void update_cell(const Params& p, const uint8_t* f2s, int j,
-                                            uint8_t f0, typename S::R m, typename S::R rho_pre,
-                                            uint32_t n, int x, int y) {
+__device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f2s, int j,
+                                               uint8_t f0, typename S::R m, typename S::R rho_pre,
+                                               uint32_t n, int x, int y, bool& interior) {
   using R = typename S::R;
   R* const mass = fld<R>(p.mass);
   const uint8_t fl = f2s[j];
+  interior = false;
   if (fl & (kWall | kMovingWall | kInlet)) {
     p.flags_next[n] = fl;
-    return;
+    return fl;
   }
   const bool cons = p.mass_mode != 0;
   const uint8_t e = fl & (kFGI | kGasInterface);
@@ -239,7 +242,7 @@ __device__ __forceinline__ void update_cell(const Params& p, const uint8_t* f2s,
       mass[n] = R(0);
       fld<R>(p.excess)[n] = R(0);
     }
-    return;
+    return fl;
   }
   // Non-gas cells had their density loaded before the flag passes; a gas cell
   // turning interface (or holding stray mass) loads it now.
@@ -320,7 +323,10 @@ __device__ __forceinline__ void update_cell(const Params& p, const uint8_t* f2s,
   // interface->fluid or gas->interface before the transitions
   // (lbmKernel2D.cu:285-292).  Conservative: the cells that pull it in the
   // next pass 1, i.e. not gas, wall or inlet after their own transition.
+  // The same settled flags tell whether all eight neighbours are fluid or
+  // interface next step (interior_fluid of the next pass 1).
   int cont = 0;
+  bool all_fi = true;
 #pragma unroll
   for (int k = 1; k < 9; k++) {
     const uint8_t q = f2s[j + lds_offset(k)];
@@ -328,9 +334,12 @@ __device__ __forceinline__ void update_cell(const Params& p, const uint8_t* f2s,
       const uint8_t r = q & (kIF | kGasInterface);
       cont += int(r == kFluid || r == kInterface || r == kIF || r == kGasInterface);
     } else {
-      cont += int((settle(q) & (kGas | kWall | kInlet)) == 0);
+      const uint8_t sq = settle(q);
+      cont += int((sq & (kGas | kWall | kInlet)) == 0);
+      all_fi = all_fi && ((sq & kFGI) == kFluid || (sq & kFGI) == kInterface);
     }
   }
+  interior = cons && all_fi;
   if (cont > 0) ex = ex / R(float(cont));
   else {
     m += ex;
@@ -340,6 +349,25 @@ __device__ __forceinline__ void update_cell(const Params& p, const uint8_t* f2s,
   fld<R>(p.excess)[n] = ex;
   if (write_phi) fld<R>(p.phi)[n] = ph;
   p.flags_next[n] = nfl;
+  return nfl;
+}
+
+// Work word of the tile for the next step (Params::tile_work): stream-collide
+// has work in a row chunk where a cell is not gas or wall, pass 1 where such a
+// cell is not an inlet and, in the conservative mode, not an interior fluid
+// cell.  One wavefront per tile row; the word is stored only when it changed
+// (most tiles keep theirs from step to step).  Block-wide: every thread of the
+// block calls it.
+__device__ __forceinline__ void write_tile_work(const Params& p, int tid, uint32_t tile,
+                                                uint32_t old, bool sc, bool fs1, uint32_t* bits,
+                                                uint32_t extra) {
+  const bool any_sc = __any(sc), any_fs1 = __any(fs1);
+  const int r = tid / 64;
+  if ((tid & 63) == 0 && (any_sc || any_fs1))
+    atomicOr(bits, (any_sc ? 1u << (kWorkSC + r) : 0u) | (any_fs1 ? 1u << (kWorkFS1 + r) : 0u));
+  __syncthreads();
+  const uint32_t w = *bits | extra;
+  if (tid == 0 && w != old) p.tile_work[tile] = w;
 }
 
 template <class S, int ODD, bool SMALL>
@@ -357,6 +385,25 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   const int x = x0 + (tid & (kTileW - 1)), y = y0 + tid / kTileW;
   const bool valid = x < nx && y < ny;
   const uint32_t n = valid ? uint32_t(x + y * pitch) : 0u;
+  const uint32_t tile = ty * p.work_x + tx;
+  const uint32_t old_work = p.tile_work[tile];
+  // A tile that was quiet in the last update (below), whose neighbour tiles
+  // held only gas and walls after it, has nothing to do: stream-collide and
+  // pass 1 skipped them, so its flags and halo are unchanged, both flag
+  // buffers hold them (the last update copied them), and no cell of the tile
+  // turned gas then.  The three-cell halo lies in the eight neighbour tiles
+  // except on the domain edges, where the test is not taken.
+  if ((old_work & kWorkQuiet) && x0 >= kHalo && x0 + kTileW + kHalo <= nx && y0 >= kHalo &&
+      y0 + kTileH + kHalo <= ny) {
+    uint32_t around = 0;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+      for (int dx = -1; dx <= 1; dx++) around |= p.tile_work[tile + dy * int(p.work_x) + dx];
+    if (!(around & kWorkSCMask)) return;
+  }
+  __shared__ uint32_t work_bits;
+  if (tid == 0) work_bits = 0;  // ordered before its use by the first barrier
   // All global loads of the tile are issued before any is used: the two halo
   // bytes of this thread, its own flag and mass.  The density of a non-gas
   // cell depends only on the own flag, so its load follows at once and
@@ -401,6 +448,8 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
         fld<R>(p.excess)[n] = R(0);
       }
     }
+    const bool sc = valid && !(own & (kGas | kWall));
+    write_tile_work(p, tid, tile, old_work, sc, sc && !(own & kInlet), &work_bits, kWorkQuiet);
     return;
   }
   for (int i = tid; i < (kLdsH - 2) * (kLdsW - 2); i += kUpdateThreads) {  // pass 2
@@ -432,10 +481,15 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
     f2s[j] = o;
   }
   __syncthreads();
+  bool sc = false, fs1 = false;
   if (valid) {
     const int j = (tid / kTileW + kHalo) * kLdsW + (tid & (kTileW - 1)) + kHalo;
-    update_cell<S, ODD>(p, f2s, j, f0s[j], m, rho_pre, n, x, y);
+    bool interior;
+    const uint8_t fin = update_cell<S, ODD>(p, f2s, j, f0s[j], m, rho_pre, n, x, y, interior);
+    sc = !(fin & (kGas | kWall));
+    fs1 = sc && !(fin & kInlet) && !(interior && (fin & (kFGI | kInlet | kOutlet)) == kFluid);
   }
+  write_tile_work(p, tid, tile, old_work, sc, fs1, &work_bits, 0u);
 }
 
 // ------------------------------------------------------------- utilities
