@@ -83,6 +83,8 @@ flb::Params params(const flb_handle* h, int bx) {
   p.tile_work = h->tile_work;
   p.tile_pub = h->tile_pub;
   p.work_x = h->work_x;
+  p.tile_rows = h->tile_rows;
+  p.tile_skip = h->tile_skip;
   return p;
 }
 
@@ -169,6 +171,24 @@ int download_field(flb_handle* h, Out* dst, const float* src, int planes) {
   int rc = cells_out(h, t.data(), src, sizeof(float), planes);
   for (size_t i = 0; i < count && !rc; ++i) dst[i] = Out(t[i]);
   return rc;
+}
+
+// Free surface: the next step evaluates everything -- every tile in the
+// update, every chunk in pass 1 and stream-collide -- after which the update
+// kernel narrows the work map again.  Used at creation and whenever the host
+// replaces cell fields.
+int reset_work(flb_handle* h) {
+  const size_t tiles = size_t(h->work_x) * h->tile_rows;
+  const size_t chunks = size_t(h->work_x) * h->ny;
+  const int rd = int(h->t & 1u);
+  std::vector<uint32_t> ids(chunks), all(tiles, 0xffffffffu);
+  for (size_t i = 0; i < ids.size(); ++i) ids[i] = uint32_t(i);
+  uint32_t counts[8] = {};
+  counts[4 * rd + flb::kListFS1] = uint32_t(chunks);
+  int rc;
+  if ((rc = upload(h, h->tile_work, all.data(), tiles * sizeof(uint32_t)))) return rc;
+  if ((rc = upload(h, h->fs1_list[rd], ids.data(), chunks * sizeof(uint32_t)))) return rc;
+  return upload(h, h->counts, counts, sizeof(counts));
 }
 
 // count: device elements.
@@ -344,6 +364,9 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   }
   if (opt->device < 0 || opt->device >= ndev) return bail(FLB_ERR_ARGUMENT);
   if (hipSetDevice(opt->device) != hipSuccess) return bail(FLB_ERR_DEVICE);
+  if (hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, opt->device) !=
+          hipSuccess || h->cus <= 0)
+    h->cus = 256;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(FLB_ERR_DEVICE);
 
@@ -366,13 +389,18 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
     if ((rc = field_alloc(h, &h->excess, Nd))) return bail(rc);
     if ((rc = field_alloc(h, &h->phi, Nd))) return bail(rc);
     if ((rc = field_alloc(h, &h->rec, 9 * Nd))) return bail(rc);
-    // Tile work words: every chunk has work before the first update kernel
-    // has written them.
+    // Work words, lists and stamps of the free-surface kernels.
     h->work_x = (nx + flb::kFsTileW - 1) / flb::kFsTileW;
-    const size_t tiles = size_t(h->work_x) * ((ny + flb::kFsTileH - 1) / flb::kFsTileH);
+    h->tile_rows = (ny + flb::kFsTileH - 1) / flb::kFsTileH;
+    const size_t tiles = size_t(h->work_x) * h->tile_rows;
+    const size_t chunks = size_t(h->work_x) * ny;
     if ((rc = dev_alloc(h, &h->tile_work, tiles))) return bail(rc);
     if ((rc = dev_alloc(h, &h->tile_pub, tiles))) return bail(rc);
-    std::vector<uint32_t> pub(tiles, 0u), all(tiles, 0xffffffffu);
+    if ((rc = dev_alloc(h, &h->tile_skip, tiles))) return bail(rc);
+    if ((rc = dev_alloc(h, &h->counts, 8))) return bail(rc);
+    for (int k = 0; k < 2; ++k)
+      if ((rc = dev_alloc(h, &h->fs1_list[k], chunks))) return bail(rc);
+    std::vector<uint32_t> pub(tiles, 0u);
     for (uint64_t n = 0; n < N; ++n) {
       const bool pub_r = n > 0 && (flags[n - 1] & FLB_INLET);
       const bool pub_v = n + 1 < N && (flags[n + 1] & FLB_OUTLET) && !(flags[n] & FLB_INLET);
@@ -382,7 +410,7 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
             1u << (flb::kWorkSC + int(y % flb::kFsTileH));
     }
     if ((rc = upload(h, h->tile_pub, pub.data(), tiles * sizeof(uint32_t)))) return bail(rc);
-    if ((rc = upload(h, h->tile_work, all.data(), tiles * sizeof(uint32_t)))) return bail(rc);
+    if ((rc = reset_work(h))) return bail(rc);
   }
   if ((rc = dev_alloc(h, &h->rho_init, Nd))) return bail(rc);
   if ((rc = dev_alloc(h, &h->u_init, 2 * Nd))) return bail(rc);
@@ -445,11 +473,7 @@ int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const 
   int rc;
   if ((rc = upload_field(h, h->mass, mass, 1))) return rc;
   if ((rc = upload_field(h, h->phi, phi, 1))) return rc;
-  {  // new masses: every tile is evaluated again in the next update
-    const size_t tiles = size_t(h->work_x) * ((h->ny + flb::kFsTileH - 1) / flb::kFsTileH);
-    std::vector<uint32_t> all(tiles, 0xffffffffu);
-    if ((rc = upload(h, h->tile_work, all.data(), tiles * sizeof(uint32_t)))) return rc;
-  }
+  if ((rc = reset_work(h))) return rc;  // new masses: evaluate everything next step
   if ((rc = zero_field(h, h->rec, h->Nd))) return rc;  // stream-collide's mass residuals
   if (excess) return upload_field(h, h->excess, excess, 1);
   return zero_field(h, h->excess, h->Nd);
@@ -472,13 +496,22 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
   };
   int rc;
   if (fs) {
-    if ((rc = run(FLB_K_FS1, flb::get_fs1(st, odd), p, bx))) return rc;
+    // Pass-1 list of this step (built by the last update) and of the next.
+    const int rd = int(h->t & 1u), wr = rd ^ 1;
+    p.fs1_list = h->fs1_list[rd];
+    p.counts_in = h->counts + 4 * rd;
+    p.fs1_list_out = h->fs1_list[wr];
+    p.counts_out = h->counts + 4 * wr;
+    flb::Params q = p;  // pass 1: a grid of resident blocks walks the list
+    q.nblocks = uint32_t(2 * h->cus);
+    if ((rc = run(FLB_K_FS1, flb::get_fs1(st, odd), q, flb::kListThreads))) return rc;
     if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, moments), p, bx)))
       return rc;
-    flb::Params q = p;  // passes 2-4: one block per kFsTileW x kFsTileH tile
-    q.tiles_x = (h->nx + flb::kFsTileW - 1) / flb::kFsTileW;
-    q.nblocks = q.tiles_x * ((h->ny + flb::kFsTileH - 1) / flb::kFsTileH);
-    if ((rc = run(FLB_K_FS_UPDATE, flb::get_fs_update(st, odd, h->nx, h->ny), q, flb::kFsTileW * flb::kFsTileH)))
+    q = p;  // passes 2-4: one block per kFsTileW x kFsTileH tile
+    q.tiles_x = h->work_x;
+    q.nblocks = h->work_x * h->tile_rows;
+    if ((rc = run(FLB_K_FS_UPDATE, flb::get_fs_update(st, odd, h->nx, h->ny), q,
+                  flb::kFsTileW * flb::kFsTileH)))
       return rc;
     h->cur ^= 1;  // the update wrote the other flag buffer
   } else {

@@ -132,8 +132,20 @@ struct Params {
   // cell that publishes for an inlet/outlet neighbour (static).
   uint32_t* tile_work;
   const uint32_t* tile_pub;
-  uint32_t work_x;
+  uint32_t work_x;     // tiles (and 64-cell chunks) per row
+  uint32_t tile_rows;  // tile rows
+  // Pass-1 work list of this step (read) and of the next step (written by the
+  // update kernel): row chunks, index = y * work_x + chunk; counts_* hold the
+  // list length at kListFS1.
+  const uint32_t* fs1_list;
+  const uint32_t* counts_in;
+  uint32_t* fs1_list_out;
+  uint32_t* counts_out;
+  uint8_t* tile_skip;  // per tile: the update of this step has nothing to do
 };
+
+enum : int { kListFS1 = 0 };
+constexpr int kListThreads = 256;  // block size of the list-driven pass 1
 
 constexpr int kWorkSC = 0, kWorkFS1 = 8;  // bit offsets in a tile_work word
 constexpr uint32_t kWorkQuiet = 1u << 16;  // the update found the tile quiet

@@ -52,7 +52,13 @@ struct flb_handle {
   float* rec = nullptr;
   uint32_t* tile_work = nullptr;  // free surface: per-tile work words (Params::tile_work)
   uint32_t* tile_pub = nullptr;   // free surface: chunks holding an inlet/outlet publisher
-  uint32_t work_x = 0;
+  uint32_t work_x = 0, tile_rows = 0;
+  // Free-surface pass-1 lists by step parity (Params::fs1_list), their
+  // lengths (4 words per parity) and the update's per-tile skip marks.
+  uint32_t* fs1_list[2] = {nullptr, nullptr};
+  uint32_t* counts = nullptr;
+  uint8_t* tile_skip = nullptr;
+  int cus = 256;  // compute units of the device (grid size of the list-driven pass 1)
   float* alpha = nullptr;     // Brinkman solid fraction (allocated on first use)
   float* rho_init = nullptr;  // rho0 and u0 as given to flb_create
   float* u_init = nullptr;

@@ -154,12 +154,6 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
   mass[n] = m;
 }
 
-template <class S, int ODD>
-__global__ void __launch_bounds__(1024) k_fs1(const Params p) {
-  const Cell c = locate(p);
-  if (!chunk_has(p, c, kWorkFS1, false)) return;
-  if (c.valid) fs1_cell<S, ODD>(p, c);
-}
 
 // Flag transition a cell applies to itself in pass 4.
 __device__ __forceinline__ uint8_t settle(uint8_t fl) {
@@ -352,58 +346,49 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
   return nfl;
 }
 
-// Work word of the tile for the next step (Params::tile_work): stream-collide
-// has work in a row chunk where a cell is not gas or wall, pass 1 where such a
-// cell is not an inlet and, in the conservative mode, not an interior fluid
-// cell.  One wavefront per tile row; the word is stored only when it changed
-// (most tiles keep theirs from step to step).  Block-wide: every thread of the
-// block calls it.
-__device__ __forceinline__ void write_tile_work(const Params& p, int tid, uint32_t tile,
-                                                uint32_t old, bool sc, bool fs1, uint32_t* bits,
-                                                uint32_t extra) {
+// End of a tile in the update kernel.  One wavefront per tile row computed
+// whether its row chunk has work next step for stream-collide (a cell that is
+// not gas or wall) and pass 1 (such a cell that is not an inlet and, in the
+// conservative mode, not an interior fluid cell).  Thread 0 stores the tile's
+// work word (Params::tile_work) if it changed and appends the chunks with
+// pass-1 work to the next step's pass-1 list.  Block-wide: every thread of
+// the block calls it.
+__device__ __forceinline__ void finish_tile(const Params& p, int tid, uint32_t tx, uint32_t ty,
+                                            uint32_t old, bool sc, bool fs1, uint32_t* bits,
+                                            uint32_t quiet) {
   const bool any_sc = __any(sc), any_fs1 = __any(fs1);
   const int r = tid / 64;
   if ((tid & 63) == 0 && (any_sc || any_fs1))
     atomicOr(bits, (any_sc ? 1u << (kWorkSC + r) : 0u) | (any_fs1 ? 1u << (kWorkFS1 + r) : 0u));
   __syncthreads();
-  const uint32_t w = *bits | extra;
-  if (tid == 0 && w != old) p.tile_work[tile] = w;
+  if (tid == 0) {
+    const uint32_t w = *bits | quiet;
+    if (w != old) p.tile_work[ty * p.work_x + tx] = w;
+    const uint32_t rows_fs1 = (w >> kWorkFS1) & 0xffu;
+    if (rows_fs1) {
+      const uint32_t chunk0 = ty * kFsTileH * p.work_x + tx;
+      uint32_t k = atomicAdd(&p.counts_out[kListFS1], uint32_t(__popc(rows_fs1)));
+      for (uint32_t m = rows_fs1; m; m &= m - 1)
+        p.fs1_list_out[k++] = chunk0 + uint32_t(__ffs(m) - 1) * p.work_x;
+    }
+  }
 }
 
+// Passes 2-4 of one tile (one kFsTileW x kFsTileH block).
 template <class S, int ODD, bool SMALL>
-__global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
+__device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32_t ty,
+                                            uint8_t* f0s, uint8_t* f1s, uint8_t* f2s,
+                                            uint32_t* work_bits, uint32_t* busy_waves) {
   using R = typename S::R;
   R* const mass = fld<R>(p.mass);
-  // f0s is padded to two bytes per thread so that every lane writes both of
-  // its halo bytes unconditionally (lanes past the tile write the padding).
-  __shared__ uint8_t f0s[2 * kUpdateThreads], f1s[kLdsH * kLdsW], f2s[kLdsH * kLdsW];
   const int nx = int(p.nx), ny = int(p.ny), pitch = int(p.pitch);
-  uint32_t tx, ty;
-  block_tile(blockIdx.x, p.tiles_x, (p.ny + kTileH - 1) / kTileH, tx, ty);
   const int x0 = int(tx) * kTileW, y0 = int(ty) * kTileH;
   const int tid = int(threadIdx.x);
   const int x = x0 + (tid & (kTileW - 1)), y = y0 + tid / kTileW;
   const bool valid = x < nx && y < ny;
   const uint32_t n = valid ? uint32_t(x + y * pitch) : 0u;
-  const uint32_t tile = ty * p.work_x + tx;
-  const uint32_t old_work = p.tile_work[tile];
-  // A tile that was quiet in the last update (below), whose neighbour tiles
-  // held only gas and walls after it, has nothing to do: stream-collide and
-  // pass 1 skipped them, so its flags and halo are unchanged, both flag
-  // buffers hold them (the last update copied them), and no cell of the tile
-  // turned gas then.  The three-cell halo lies in the eight neighbour tiles
-  // except on the domain edges, where the test is not taken.
-  if ((old_work & kWorkQuiet) && x0 >= kHalo && x0 + kTileW + kHalo <= nx && y0 >= kHalo &&
-      y0 + kTileH + kHalo <= ny) {
-    uint32_t around = 0;
-#pragma unroll
-    for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-      for (int dx = -1; dx <= 1; dx++) around |= p.tile_work[tile + dy * int(p.work_x) + dx];
-    if (!(around & kWorkSCMask)) return;
-  }
-  __shared__ uint32_t work_bits;
-  if (tid == 0) work_bits = 0;  // ordered before its use by the first barrier
+  const uint32_t old_work = p.tile_work[ty * p.work_x + tx];
+  if (tid == 0) *work_bits = 0;  // ordered before its use by the next barrier
   // All global loads of the tile are issued before any is used: the two halo
   // bytes of this thread, its own flag and mass.  The density of a non-gas
   // cell depends only on the own flag, so its load follows at once and
@@ -429,17 +414,16 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   f0s[i1] = h1;
   // A flag changes in passes 2-4 only within three cells of a fluid or
   // interface cell, and only a gas cell holding mass needs more than its flag
-  // copied.  Tiles without either (most of a dam-break domain) skip the passes.
+  // copied.  Tiles without either skip the passes.
   constexpr uint8_t kWet = kFluid | kInterface | kGasInterface;
   const bool own_plain = !valid || (own & (kWall | kMovingWall | kInlet)) ||
                          ((own & (kFGI | kGasInterface)) == kGas && m == R(0));
   const bool busy = (h0 & kWet) || (has1 && (h1 & kWet)) || !own_plain;
-  __shared__ uint64_t busy_waves;  // one byte per wavefront
-  static_assert(kUpdateThreads / 64 <= 8, "one byte per wavefront");
   const bool wave_busy = __any(busy);
-  if ((tid & 63) == 0) reinterpret_cast<uint8_t*>(&busy_waves)[tid >> 6] = wave_busy;
+  if ((tid & 63) == 0) reinterpret_cast<uint8_t*>(busy_waves)[tid >> 6] = wave_busy;
   __syncthreads();
-  if (!busy_waves) {
+  const bool quiet = (busy_waves[0] | busy_waves[1]) == 0;
+  if (quiet) {
     if (valid) {
       p.flags_next[n] = own;
       if (!(own & (kWall | kMovingWall | kInlet)) && (own & (kFGI | kGasInterface)) == kGas &&
@@ -449,7 +433,7 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
       }
     }
     const bool sc = valid && !(own & (kGas | kWall));
-    write_tile_work(p, tid, tile, old_work, sc, sc && !(own & kInlet), &work_bits, kWorkQuiet);
+    finish_tile(p, tid, tx, ty, old_work, sc, sc && !(own & kInlet), work_bits, kWorkQuiet);
     return;
   }
   for (int i = tid; i < (kLdsH - 2) * (kLdsW - 2); i += kUpdateThreads) {  // pass 2
@@ -489,7 +473,64 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
     sc = !(fin & (kGas | kWall));
     fs1 = sc && !(fin & kInlet) && !(interior && (fin & (kFGI | kInlet | kOutlet)) == kFluid);
   }
-  write_tile_work(p, tid, tile, old_work, sc, fs1, &work_bits, 0u);
+  finish_tile(p, tid, tx, ty, old_work, sc, fs1, work_bits, 0u);
+}
+
+// The update kernel: one block per tile.  A tile that pass 1 of this step
+// found idle (Params::tile_skip) leaves at once: it was quiet in its last
+// update and its neighbour tiles held only gas and walls after the last step,
+// so stream-collide and pass 1 left them alone, its flags and halo are
+// unchanged, both flag buffers hold them and no cell of it turned gas then.
+template <class S, int ODD, bool SMALL>
+__global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
+  uint32_t tx, ty;
+  block_tile(blockIdx.x, p.tiles_x, p.tile_rows, tx, ty);
+  if (p.tile_skip[ty * p.work_x + tx]) return;
+  // f0s is padded to two bytes per thread so that every lane writes both of
+  // its halo bytes unconditionally (lanes past the tile write the padding).
+  __shared__ uint8_t f0s[2 * kUpdateThreads], f1s[kLdsH * kLdsW], f2s[kLdsH * kLdsW];
+  __shared__ uint32_t work_bits;
+  __shared__ uint32_t busy_waves[2];  // one byte per wavefront
+  static_assert(kUpdateThreads / 64 <= 8, "one byte per wavefront");
+  update_tile<S, ODD, SMALL>(p, tx, ty, f0s, f1s, f2s, &work_bits, busy_waves);
+}
+
+// Pass 1 over the chunk list of this step (one wavefront per 64-cell row
+// chunk), by a grid of resident blocks.  It also empties the pass-1 list the
+// update kernel of this step fills (its last readers, the kernels of the
+// previous step, have finished), and marks the tiles the update may skip:
+// quiet in their last update (kWorkQuiet), not on the domain edge -- where the
+// three-cell halo may reach past the neighbour tiles -- and with only gas and
+// walls in the 3 x 3 tiles around them after the last step.
+template <class S, int ODD>
+__global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) p.counts_out[kListFS1] = 0;
+  const uint32_t count = p.counts_in[kListFS1];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t nw = gridDim.x * (blockDim.x / 64u);
+  for (uint32_t i = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; i < count; i += nw) {
+    const uint32_t c = p.fs1_list[i];
+    const uint32_t y = c / p.work_x, x = (c - y * p.work_x) * kFsTileW + lane;
+    const Cell cell{x, y, x + y * p.pitch, x < p.nx};
+    if (cell.valid) fs1_cell<S, ODD>(p, cell);
+  }
+  const uint32_t tiles = p.work_x * p.tile_rows;
+  const uint32_t nt = gridDim.x * blockDim.x;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += nt) {
+    const uint32_t ty = t / p.work_x, tx = t - ty * p.work_x;
+    const int x0 = int(tx) * kTileW, y0 = int(ty) * kTileH;
+    bool skip = (p.tile_work[t] & kWorkQuiet) && !p.tile_pub[t] && x0 >= kHalo &&
+                x0 + kTileW + kHalo <= int(p.nx) && y0 >= kHalo && y0 + kTileH + kHalo <= int(p.ny);
+    if (skip) {
+      uint32_t around = 0;
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) around |= p.tile_work[t + dy * int(p.work_x) + dx];
+      skip = !(around & kWorkSCMask);
+    }
+    p.tile_skip[t] = skip;
+  }
 }
 
 // ------------------------------------------------------------- utilities
