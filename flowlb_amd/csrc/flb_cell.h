@@ -70,15 +70,17 @@ __device__ __forceinline__ Cell locate(const Params& p) {
 // chunks of gas and walls.  The chunk is uniform over the wavefront (a block
 // is one row segment that starts at a multiple of 64), so its tile word is a
 // scalar load.
-__device__ __forceinline__ bool chunk_has(const Params& p, const Cell& c, int base,
-                                          bool with_pub) {
+// Returns the work bits of the chunk (bit kWorkSC: stream-collide, bit
+// kWorkFS1: pass 1), 0 for a wavefront past the row's end.
+__device__ __forceinline__ uint32_t chunk_work(const Params& p, const Cell& c, bool with_pub) {
   const uint32_t tx = uint32_t(__builtin_amdgcn_readfirstlane(c.x)) / kFsTileW;
   const uint32_t y = uint32_t(__builtin_amdgcn_readfirstlane(c.y));
-  if (tx >= p.work_x) return false;  // the wavefront lies past the row's end
+  if (tx >= p.work_x) return 0u;
   const uint32_t i = (y / kFsTileH) * p.work_x + tx;
-  uint32_t w = p.tile_work[i];
-  if (with_pub) w |= p.tile_pub[i];
-  return (w >> (base + int(y % kFsTileH))) & 1u;
+  const int r = int(y % kFsTileH);
+  const uint32_t w = p.tile_work[i];
+  const uint32_t sc = (w | (with_pub ? p.tile_pub[i] : 0u)) >> (kWorkSC + r);
+  return (sc & 1u) << kWorkSC | ((w >> (kWorkFS1 + r)) & 1u) << kWorkFS1;
 }
 
 // Periodic neighbour indices (src/cuda/cellCalculations.cuh:82-109) without
@@ -199,8 +201,12 @@ __device__ __forceinline__ void store_outgoing(const Params& p, const uint32_t* 
 // the free-surface passes read them).  TRACE: record (rho, m_x, m_y) of every
 // cell -- or, for inlet/outlet cells, the density and velocity they read -- in
 // p.trace for the adjoint sweep.
+// dense (free surface, uniform over the wavefront): the chunk has no pass-1
+// work, so every fluid cell in it is an interior fluid cell and none is an
+// interface cell -- the neighbour flags need not be loaded.
 template <class S, int ODD, bool FS, bool MOM, bool TRACE>
-__device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell c) {
+__device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell c,
+                                                    bool dense = false) {
   using R = typename S::R;
   R* const rho_f = fld<R>(p.rho);
   R* const u_f = fld<R>(p.u);
@@ -245,7 +251,14 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   // 16-bit storage rounds every stored population; that rounding changes the
   // population mass, which the interior-fluid shortcut below would take over.
   constexpr bool kRounded = S::kBytes < 4;
-  if (FS) neighbour_flags(p, nb, nf);
+  if (FS) {
+    if (dense) {
+#pragma unroll
+      for (int i = 1; i < 9; i++) nf[i] = kFluid;
+    } else {
+      neighbour_flags(p, nb, nf);
+    }
+  }
   if (fluid_dev) {
 #pragma unroll
     for (int i = 1; i < 9; i++) ex[i] = fld<R>(p.excess)[nb[i]];

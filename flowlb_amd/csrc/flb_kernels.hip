@@ -28,8 +28,13 @@ namespace flb {
 template <class S, int ODD, bool FS, bool MOM, bool TRACE = false>
 __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
   const Cell c = locate(p);
-  if (FS && !chunk_has(p, c, kWorkSC, true)) return;
-  if (c.valid) stream_collide_cell<S, ODD, FS, MOM, TRACE>(p, c);
+  uint32_t work = 0;
+  if (FS) {
+    work = chunk_work(p, c, true);
+    if (!(work & (1u << kWorkSC))) return;
+  }
+  const bool dense = FS && p.mass_mode != 0 && !(work & (1u << kWorkFS1));
+  if (c.valid) stream_collide_cell<S, ODD, FS, MOM, TRACE>(p, c, dense);
 }
 
 // ------------------------------------------------------------- free surface
@@ -348,8 +353,10 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
 
 // End of a tile in the update kernel.  One wavefront per tile row computed
 // whether its row chunk has work next step for stream-collide (a cell that is
-// not gas or wall) and pass 1 (such a cell that is not an inlet and, in the
-// conservative mode, not an interior fluid cell).  Thread 0 stores the tile's
+// not gas or wall) and pass 1 (such a cell that is not, in the conservative
+// mode, an interior fluid cell; inlets are included although pass 1 passes
+// them by, so that a chunk without pass-1 work holds only interior fluid
+// cells besides gas and walls, which stream-collide relies on).  Thread 0 stores the tile's
 // work word (Params::tile_work) if it changed and appends the chunks with
 // pass-1 work to the next step's pass-1 list.  Block-wide: every thread of
 // the block calls it.
@@ -433,7 +440,7 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
       }
     }
     const bool sc = valid && !(own & (kGas | kWall));
-    finish_tile(p, tid, tx, ty, old_work, sc, sc && !(own & kInlet), work_bits, kWorkQuiet);
+    finish_tile(p, tid, tx, ty, old_work, sc, sc, work_bits, kWorkQuiet);
     return;
   }
   for (int i = tid; i < (kLdsH - 2) * (kLdsW - 2); i += kUpdateThreads) {  // pass 2
@@ -471,7 +478,7 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
     bool interior;
     const uint8_t fin = update_cell<S, ODD>(p, f2s, j, f0s[j], m, rho_pre, n, x, y, interior);
     sc = !(fin & (kGas | kWall));
-    fs1 = sc && !(fin & kInlet) && !(interior && (fin & (kFGI | kInlet | kOutlet)) == kFluid);
+    fs1 = sc && !(interior && (fin & (kFGI | kInlet | kOutlet)) == kFluid);
   }
   finish_tile(p, tid, tx, ty, old_work, sc, fs1, work_bits, 0u);
 }
