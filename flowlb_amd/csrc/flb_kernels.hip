@@ -426,10 +426,28 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
   const bool own_plain = !valid || (own & (kWall | kMovingWall | kInlet)) ||
                          ((own & (kFGI | kGasInterface)) == kGas && m == R(0));
   const bool busy = (h0 & kWet) || (has1 && (h1 & kWet)) || !own_plain;
-  const bool wave_busy = __any(busy);
-  if ((tid & 63) == 0) reinterpret_cast<uint8_t*>(busy_waves)[tid >> 6] = wave_busy;
+  // Inside a body of fluid (every flag of the tile and its halo plain fluid)
+  // passes 2 and 3 change nothing and every cell has eight receivers.
+  const bool thin = h0 != kFluid || (has1 && h1 != kFluid);
+  const uint32_t wave_bits = (__any(busy) ? 1u : 0u) | (__any(thin) ? 2u : 0u);
+  if ((tid & 63) == 0) reinterpret_cast<uint8_t*>(busy_waves)[tid >> 6] = uint8_t(wave_bits);
   __syncthreads();
-  const bool quiet = (busy_waves[0] | busy_waves[1]) == 0;
+  const uint32_t any_bits = busy_waves[0] | busy_waves[1];
+  const bool quiet = (any_bits & 0x01010101u) == 0;
+  const bool body = (any_bits & 0x02020202u) == 0;
+  if (body) {
+    // The fluid branch of update_cell with cont = 8: the excess (the
+    // deviation in the conservative mode) is shared by the eight neighbours.
+    const bool cons = p.mass_mode != 0;
+    if (valid) {
+      const R ex = cons ? m : m - rho_pre;
+      mass[n] = rho_pre;
+      fld<R>(p.excess)[n] = ex / R(8.0f);
+      p.flags_next[n] = own;
+    }
+    finish_tile(p, tid, tx, ty, old_work, valid, valid && !cons, work_bits, 0u);
+    return;
+  }
   if (quiet) {
     if (valid) {
       p.flags_next[n] = own;

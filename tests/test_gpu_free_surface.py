@@ -62,6 +62,23 @@ def test_dam_break_parity(orc, mass_mode):
     assert du < 1e-3 and dm < 1e-2
 
 
+def test_dam_break_with_fluid_body_parity(orc):
+    """512 x 128 dam break (water 225 x 75): the update kernel's 64 x 8 tiles
+    inside the water body, whose tile and halo are all plain fluid, take its
+    body-of-fluid path; the front and surface tiles take the general one."""
+    case = dataclasses.replace(cases.scaled(cases.FS_4096, 8), name="dam_break_512x128_fast",
+                               force=(0.0, -2e-4), tau=0.6)
+    g, r, flags0 = run_both(orc, case, 300, 1)
+    same = (g["flags"] == r["flags"]).mean()
+    du = np.abs(g["u"] - r["u"]).max()
+    dm = np.abs(g["mass"] - r["mass"]).max()
+    print(f"\nfs 512x128 x 300: flags equal {same:.5f} (cells changed "
+          f"{(r['flags'] != flags0).sum()}) max|du| {du:.2e} max|dm| {dm:.2e}")
+    assert (r["flags"] != flags0).sum() > 100
+    assert same >= 0.999
+    assert du < 1e-3 and dm < 1e-2
+
+
 def test_conservation_fp32_1e4_steps():
     """Conservative mode on the GPU: |M(t) - M(0)| / M(0) <= 1e-7 over 10^4 steps."""
     case = cases.scaled(cases.FS_4096, 8)      # 512 x 128
