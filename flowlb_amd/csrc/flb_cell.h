@@ -201,12 +201,13 @@ __device__ __forceinline__ void store_outgoing(const Params& p, const uint32_t* 
 // the free-surface passes read them).  TRACE: record (rho, m_x, m_y) of every
 // cell -- or, for inlet/outlet cells, the density and velocity they read -- in
 // p.trace for the adjoint sweep.
-// dense (free surface, uniform over the wavefront): the chunk has no pass-1
-// work, so every fluid cell in it is an interior fluid cell and none is an
-// interface cell -- the neighbour flags need not be loaded.
-template <class S, int ODD, bool FS, bool MOM, bool TRACE>
-__device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell c,
-                                                    bool dense = false) {
+// DENSE (free surface, conservative mode): the cell lies in a row chunk
+// without pass-1 work, so it is gas, wall or an interior fluid cell -- no
+// interface, inlet or outlet cell -- and the neighbour flags need not be
+// loaded.  Compiling that case separately keeps the kernel that runs the
+// bodies of fluid to fewer registers.
+template <class S, int ODD, bool FS, bool MOM, bool TRACE, bool DENSE = false>
+__device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell c) {
   using R = typename S::R;
   R* const rho_f = fld<R>(p.rho);
   R* const u_f = fld<R>(p.u);
@@ -245,20 +246,13 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   // with the populations, so that the cell waits for memory once.
   uint8_t nf[9];
   R ex[9], rc[9], m_if = R(0), resid = R(0);
-  const bool cons = FS && p.mass_mode != 0;
-  const bool fluid_dev = cons && (fl & (kFGI | kInlet | kOutlet)) == kFluid;
-  const bool iface = FS && (fl & kFGI) == kInterface;
+  const bool cons = FS && (DENSE || p.mass_mode != 0);
+  const bool fluid_dev = cons && (DENSE || (fl & (kFGI | kInlet | kOutlet)) == kFluid);
+  const bool iface = FS && !DENSE && (fl & kFGI) == kInterface;
   // 16-bit storage rounds every stored population; that rounding changes the
   // population mass, which the interior-fluid shortcut below would take over.
   constexpr bool kRounded = S::kBytes < 4;
-  if (FS) {
-    if (dense) {
-#pragma unroll
-      for (int i = 1; i < 9; i++) nf[i] = kFluid;
-    } else {
-      neighbour_flags(p, nb, nf);
-    }
-  }
+  if (FS && !DENSE) neighbour_flags(p, nb, nf);
   if (fluid_dev) {
 #pragma unroll
     for (int i = 1; i < 9; i++) ex[i] = fld<R>(p.excess)[nb[i]];
@@ -277,7 +271,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
       if (gate == kGas) g[k] = rc[k];
     }
   }
-  const uint8_t io = fl & (kInlet | kOutlet);
+  const uint8_t io = DENSE ? 0 : fl & (kInlet | kOutlet);
   R rho, rho1, ux, uy;
   if (io & kInlet) {
     rho = prho1[lin.next];
@@ -289,7 +283,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
       p.trace[N + n] = ux;
       p.trace[2 * N + n] = uy;
     }
-  } else if (fl & kOutlet) {
+  } else if (io & kOutlet) {
     rho = rho_f[n];
     rho1 = rho - R(1);
     ux = pu1[lin.prev];
@@ -314,7 +308,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
       p.trace[2 * N + n] = my;
     }
   }
-  if (fluid_dev && interior_fluid(fl, nf)) {
+  if (fluid_dev && (DENSE || interior_fluid(fl, nf))) {
     // Deviation m - rho that link-by-link accounting would give: the pulled
     // shares, plus the rounding of rho itself, plus (16-bit storage) the mass
     // the previous step's stores rounded away.
@@ -340,7 +334,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   }
   R fi[9];
   bool forced = false;
-  if (p.alpha && !io) {
+  if (!FS && p.alpha && !io) {  // Brinkman drag is offered for monofluid handles only
     // Brinkman drag G = F - alpha rho u with the Guo half-force shift solved
     // for u (no reference counterpart; FP32 storage only).
     const R a = R(p.alpha[n]);
@@ -363,14 +357,14 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     // of a fluid cell only at readback -- pass 1 reads the velocity of
     // inlet/outlet interface cells, pass 3 averages interface neighbours
     // (a fluid cell never neighbours a gas or gas->interface cell).
-    const bool want_u = MOM || (fl & kFGI) != kFluid;
+    const bool want_u = MOM || (!DENSE && (fl & kFGI) != kFluid);
     if (!io) {
       rho_f[n] = rho;
       if (want_u) {
         u_f[n] = ux;
         u_f[N + n] = uy;
       }
-    } else if (fl & kOutlet) {
+    } else if (io & kOutlet) {
       u_f[n] = ux;
       u_f[N + n] = uy;
     }
@@ -383,7 +377,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   }
   R fe[9];
   equilibrium_shifted<R>(rho, rho1, ux, uy, fe);
-  if (FS && p.mass_mode != 0) {
+  if (FS && (DENSE || p.mass_mode != 0)) {
     // The float weights sum to 1 + 7.45e-9, so every collision adds
     // omega * 7.45e-9 * (rho - 1) of population mass; interior fluid cells take
     // their mass from the populations, which turned this into a drift of

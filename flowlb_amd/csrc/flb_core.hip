@@ -505,8 +505,18 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
     flb::Params q = p;  // pass 1: a grid of resident blocks walks the list
     q.nblocks = uint32_t(2 * h->cus);
     if ((rc = run(FLB_K_FS1, flb::get_fs1(st, odd), q, flb::kListThreads))) return rc;
-    if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, moments), p, bx)))
-      return rc;
+    if (h->opt.mass_mode != FLB_MASS_CONSERVATIVE) {
+      if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, moments), p, bx)))
+        return rc;
+    } else {
+      // Bodies of fluid in one launch over the grid, the surface chunks (the
+      // pass-1 list) in a second one.
+      if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE], h->stream));
+      if ((rc = launch(h, flb::get_stream_collide_dense(st, odd, moments), p, bx))) return rc;
+      if ((rc = launch(h, flb::get_stream_collide_surface(st, odd, moments), q, flb::kListThreads)))
+        return rc;
+      if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE + 1], h->stream));
+    }
     q = p;  // passes 2-4: one block per kFsTileW x kFsTileH tile
     q.tiles_x = h->work_x;
     q.nblocks = h->work_x * h->tile_rows;

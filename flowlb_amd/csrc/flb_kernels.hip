@@ -25,16 +25,19 @@ namespace flb {
 // One block per tile.  (A grid of resident blocks looping over the tiles was
 // measured 30 % slower on the 8192 x 2048 dam break: the hardware overlaps the
 // short all-gas wavefronts of independent blocks better than a loop does.)
-template <class S, int ODD, bool FS, bool MOM, bool TRACE = false>
+// Free surface: a wavefront whose row chunk has no stream-collide work leaves
+// after reading its tile word.  DENSE (conservative mode): the kernel runs only
+// the chunks without pass-1 work -- gas, walls and interior fluid -- and
+// k_stream_collide_surface the others.
+template <class S, int ODD, bool FS, bool MOM, bool TRACE = false, bool DENSE = false>
 __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
   const Cell c = locate(p);
-  uint32_t work = 0;
   if (FS) {
-    work = chunk_work(p, c, true);
+    const uint32_t work = chunk_work(p, c, true);
     if (!(work & (1u << kWorkSC))) return;
+    if (DENSE && (work & (1u << kWorkFS1))) return;
   }
-  const bool dense = FS && p.mass_mode != 0 && !(work & (1u << kWorkFS1));
-  if (c.valid) stream_collide_cell<S, ODD, FS, MOM, TRACE>(p, c, dense);
+  if (c.valid) stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE>(p, c);
 }
 
 // ------------------------------------------------------------- free surface
@@ -558,6 +561,22 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
   }
 }
 
+// Free-surface stream-collide of the chunks with pass-1 work (the surface,
+// inlets and outlets), from the pass-1 list, by a grid of resident blocks;
+// k_stream_collide<..., DENSE> runs the rest.
+template <class S, int ODD, bool MOM>
+__global__ void __launch_bounds__(kListThreads) k_stream_collide_surface(const Params p) {
+  const uint32_t count = p.counts_in[kListFS1];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t nw = gridDim.x * (blockDim.x / 64u);
+  for (uint32_t i = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; i < count; i += nw) {
+    const uint32_t c = p.fs1_list[i];
+    const uint32_t y = c / p.work_x, x = (c - y * p.work_x) * kFsTileW + lane;
+    const Cell cell{x, y, x + y * p.pitch, x < p.nx};
+    if (cell.valid) stream_collide_cell<S, ODD, true, MOM, false>(p, cell);
+  }
+}
+
 // ------------------------------------------------------------- utilities
 __device__ __forceinline__ double block_sum(double v, double* lds) {
 #pragma unroll
@@ -670,6 +689,19 @@ KernelFn stream_collide_fn(int odd, bool fs, bool mom) {
   return odd ? k_stream_collide<S, 1, false, false> : k_stream_collide<S, 0, false, false>;
 }
 template <class S>
+KernelFn dense_fn(int odd, bool mom) {
+  if (mom)
+    return odd ? k_stream_collide<S, 1, true, true, false, true>
+               : k_stream_collide<S, 0, true, true, false, true>;
+  return odd ? k_stream_collide<S, 1, true, false, false, true>
+             : k_stream_collide<S, 0, true, false, false, true>;
+}
+template <class S>
+KernelFn surface_fn(int odd, bool mom) {
+  if (mom) return odd ? k_stream_collide_surface<S, 1, true> : k_stream_collide_surface<S, 0, true>;
+  return odd ? k_stream_collide_surface<S, 1, false> : k_stream_collide_surface<S, 0, false>;
+}
+template <class S>
 KernelFn fs1_fn(int odd) { return odd ? k_fs1<S, 1> : k_fs1<S, 0>; }
 template <class S>
 KernelFn fs_update_fn(int odd, bool small) {
@@ -697,6 +729,22 @@ KernelFn get_stream_collide(int storage, int odd, bool fs, bool mom) {
     case 2: return stream_collide_fn<StoreBF16S>(odd, fs, mom);
     case 3: return stream_collide_fn<StoreF64>(odd, fs, mom);
     default: return stream_collide_fn<StoreF32>(odd, fs, mom);
+  }
+}
+KernelFn get_stream_collide_dense(int storage, int odd, bool mom) {
+  switch (storage) {
+    case 1: return dense_fn<StoreF16S>(odd, mom);
+    case 2: return dense_fn<StoreBF16S>(odd, mom);
+    case 3: return dense_fn<StoreF64>(odd, mom);
+    default: return dense_fn<StoreF32>(odd, mom);
+  }
+}
+KernelFn get_stream_collide_surface(int storage, int odd, bool mom) {
+  switch (storage) {
+    case 1: return surface_fn<StoreF16S>(odd, mom);
+    case 2: return surface_fn<StoreBF16S>(odd, mom);
+    case 3: return surface_fn<StoreF64>(odd, mom);
+    default: return surface_fn<StoreF32>(odd, mom);
   }
 }
 KernelFn get_fs1(int storage, int odd) {
