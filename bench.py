@@ -311,7 +311,9 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "stream_collide",
+                     "traffic": traffic,
+                     "kernel": "stream_collide" + (" (bodies-of-fluid + surface launches, timed "
+                                                   "as one)" if case.free_surface else ""),
                      "algorithmic_bytes_per_launch": int(r["sc_bytes"]),
                      "avg_launch_ms": round(r["sc_ms"], 5)},
         "mlups_active_cells": round(r["active"] / (ms_step * 1e-3) / 1e6, 1),
