@@ -84,7 +84,7 @@ flb::Params params(const flb_handle* h, int bx) {
   p.tile_pub = h->tile_pub;
   p.work_x = h->work_x;
   p.tile_rows = h->tile_rows;
-  p.tile_skip = h->tile_skip;
+  p.tile_list = h->tile_list;
   return p;
 }
 
@@ -185,9 +185,11 @@ int reset_work(flb_handle* h) {
   for (size_t i = 0; i < ids.size(); ++i) ids[i] = uint32_t(i);
   uint32_t counts[8] = {};
   counts[4 * rd + flb::kListFS1] = uint32_t(chunks);
+  const uint32_t zero2[2] = {0u, 0u};
   int rc;
   if ((rc = upload(h, h->tile_work, all.data(), tiles * sizeof(uint32_t)))) return rc;
   if ((rc = upload(h, h->fs1_list[rd], ids.data(), chunks * sizeof(uint32_t)))) return rc;
+  if ((rc = upload(h, h->tile_counts, zero2, sizeof(zero2)))) return rc;
   return upload(h, h->counts, counts, sizeof(counts));
 }
 
@@ -396,7 +398,8 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
     const size_t chunks = size_t(h->work_x) * ny;
     if ((rc = dev_alloc(h, &h->tile_work, tiles))) return bail(rc);
     if ((rc = dev_alloc(h, &h->tile_pub, tiles))) return bail(rc);
-    if ((rc = dev_alloc(h, &h->tile_skip, tiles))) return bail(rc);
+    if ((rc = dev_alloc(h, &h->tile_list, tiles))) return bail(rc);
+    if ((rc = dev_alloc(h, &h->tile_counts, 2))) return bail(rc);
     if ((rc = dev_alloc(h, &h->counts, 8))) return bail(rc);
     for (int k = 0; k < 2; ++k)
       if ((rc = dev_alloc(h, &h->fs1_list[k], chunks))) return bail(rc);
@@ -502,6 +505,8 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
     p.counts_in = h->counts + 4 * rd;
     p.fs1_list_out = h->fs1_list[wr];
     p.counts_out = h->counts + 4 * wr;
+    p.tile_count = h->tile_counts + rd;
+    p.tile_count_next = h->tile_counts + wr;
     flb::Params q = p;  // pass 1: a grid of resident blocks walks the list
     q.nblocks = uint32_t(2 * h->cus);
     if ((rc = run(FLB_K_FS1, flb::get_fs1(st, odd), q, flb::kListThreads))) return rc;
@@ -517,9 +522,9 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
         return rc;
       if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE + 1], h->stream));
     }
-    q = p;  // passes 2-4: one block per kFsTileW x kFsTileH tile
+    q = p;  // passes 2-4: blocks of kFsTileW x kFsTileH threads walk the tile list
     q.tiles_x = h->work_x;
-    q.nblocks = h->work_x * h->tile_rows;
+    q.nblocks = uint32_t(flb::kUpdateBlocksPerCU * h->cus);
     if ((rc = run(FLB_K_FS_UPDATE, flb::get_fs_update(st, odd, h->nx, h->ny), q,
                   flb::kFsTileW * flb::kFsTileH)))
       return rc;

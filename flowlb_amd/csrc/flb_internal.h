@@ -59,7 +59,8 @@ struct flb_handle {
   // lengths (4 words per parity) and the update's per-tile skip marks.
   uint32_t* fs1_list[2] = {nullptr, nullptr};
   uint32_t* counts = nullptr;
-  uint8_t* tile_skip = nullptr;
+  uint32_t* tile_list = nullptr;  // tiles the update of this step evaluates
+  uint32_t* tile_counts = nullptr;  // its length, by step parity
   int cus = 256;  // compute units of the device (grid size of the list-driven pass 1)
   float* alpha = nullptr;     // Brinkman solid fraction (allocated on first use)
   float* rho_init = nullptr;  // rho0 and u0 as given to flb_create

@@ -504,35 +504,46 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
   finish_tile(p, tid, tx, ty, old_work, sc, fs1, work_bits, 0u);
 }
 
-// The update kernel: one block per tile.  A tile that pass 1 of this step
-// found idle (Params::tile_skip) leaves at once: it was quiet in its last
-// update and its neighbour tiles held only gas and walls after the last step,
-// so stream-collide and pass 1 left them alone, its flags and halo are
-// unchanged, both flag buffers hold them and no cell of it turned gas then.
+// The update kernel walks the tile list pass 1 built for this step.  A tile
+// left off the list was quiet in its last update and its neighbour tiles held
+// only gas and walls after the last step, so stream-collide and pass 1 left
+// them alone, its flags and halo are unchanged, both flag buffers hold them
+// and no cell of it turned gas then: it has nothing to do.  The barrier at
+// the end of finish_tile orders a tile's LDS reads before the next tile's
+// writes; the next tile's index is loaded while the current one is evaluated.
 template <class S, int ODD, bool SMALL>
 __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
-  uint32_t tx, ty;
-  block_tile(blockIdx.x, p.tiles_x, p.tile_rows, tx, ty);
-  if (p.tile_skip[ty * p.work_x + tx]) return;
   // f0s is padded to two bytes per thread so that every lane writes both of
   // its halo bytes unconditionally (lanes past the tile write the padding).
   __shared__ uint8_t f0s[2 * kUpdateThreads], f1s[kLdsH * kLdsW], f2s[kLdsH * kLdsW];
   __shared__ uint32_t work_bits;
   __shared__ uint32_t busy_waves[2];  // one byte per wavefront
   static_assert(kUpdateThreads / 64 <= 8, "one byte per wavefront");
-  update_tile<S, ODD, SMALL>(p, tx, ty, f0s, f1s, f2s, &work_bits, busy_waves);
+  const uint32_t count = *p.tile_count;
+  uint32_t i = blockIdx.x;
+  uint32_t tile = i < count ? p.tile_list[i] : 0u;
+  while (i < count) {
+    const uint32_t j = i + gridDim.x;
+    const uint32_t next = j < count ? p.tile_list[j] : 0u;
+    const uint32_t ty = tile / p.work_x, tx = tile - ty * p.work_x;
+    update_tile<S, ODD, SMALL>(p, tx, ty, f0s, f1s, f2s, &work_bits, busy_waves);
+    i = j;
+    tile = next;
+  }
 }
 
 // Pass 1 over the chunk list of this step (one wavefront per 64-cell row
 // chunk), by a grid of resident blocks.  It also empties the pass-1 list the
-// update kernel of this step fills (its last readers, the kernels of the
-// previous step, have finished), and marks the tiles the update may skip:
+// update kernel of this step fills and the counter of the next step's tile
+// list (their last readers, the kernels of the previous step, have finished),
+// and lists the tiles the update of this step must evaluate: all but those
 // quiet in their last update (kWorkQuiet), not on the domain edge -- where the
 // three-cell halo may reach past the neighbour tiles -- and with only gas and
 // walls in the 3 x 3 tiles around them after the last step.
 template <class S, int ODD>
 __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
   if (blockIdx.x == 0 && threadIdx.x == 0) p.counts_out[kListFS1] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 1) *p.tile_count_next = 0;
   const uint32_t count = p.counts_in[kListFS1];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nw = gridDim.x * (blockDim.x / 64u);
@@ -544,20 +555,32 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
   }
   const uint32_t tiles = p.work_x * p.tile_rows;
   const uint32_t nt = gridDim.x * blockDim.x;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += nt) {
-    const uint32_t ty = t / p.work_x, tx = t - ty * p.work_x;
-    const int x0 = int(tx) * kTileW, y0 = int(ty) * kTileH;
-    bool skip = (p.tile_work[t] & kWorkQuiet) && !p.tile_pub[t] && x0 >= kHalo &&
-                x0 + kTileW + kHalo <= int(p.nx) && y0 >= kHalo && y0 + kTileH + kHalo <= int(p.ny);
-    if (skip) {
-      uint32_t around = 0;
+  // One tile per lane; the loop is uniform over the wavefront (ballots).
+  for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < tiles; t0 += nt) {
+    const uint32_t t = t0 + lane;
+    bool need = false;
+    if (t < tiles) {
+      const uint32_t ty = t / p.work_x, tx = t - ty * p.work_x;
+      const int x0 = int(tx) * kTileW, y0 = int(ty) * kTileH;
+      bool skip = (p.tile_work[t] & kWorkQuiet) && !p.tile_pub[t] && x0 >= kHalo &&
+                  x0 + kTileW + kHalo <= int(p.nx) && y0 >= kHalo &&
+                  y0 + kTileH + kHalo <= int(p.ny);
+      if (skip) {
+        uint32_t around = 0;
 #pragma unroll
-      for (int dy = -1; dy <= 1; dy++)
+        for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
-        for (int dx = -1; dx <= 1; dx++) around |= p.tile_work[t + dy * int(p.work_x) + dx];
-      skip = !(around & kWorkSCMask);
+          for (int dx = -1; dx <= 1; dx++) around |= p.tile_work[t + dy * int(p.work_x) + dx];
+        skip = !(around & kWorkSCMask);
+      }
+      need = !skip;
     }
-    p.tile_skip[t] = skip;
+    // One atomic per wavefront appends its listed tiles.
+    const uint64_t m = __ballot(need);
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(p.tile_count, uint32_t(__popcll(m)));
+    base = __shfl(base, 0, 64);
+    if (need) p.tile_list[base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)))] = t;
   }
 }
 
