@@ -155,7 +155,8 @@ void drop_graph(flb_batch* b) {
 }
 
 int launch_step(flb_batch* b, int odd, bool mom) {
-  hipLaunchKernelGGL(flb::get_batch(b->storage, odd, mom), dim3(b->grid), dim3(b->block_x), 0,
+  const uint32_t grid = b->grid > 0 ? uint32_t(b->grid) : b->ntiles;  // 0: a block per tile
+  hipLaunchKernelGGL(flb::get_batch(b->storage, odd, mom), dim3(grid), dim3(b->block_x), 0,
                      b->stream, b->tiles, b->ntiles, b->inst);
   BAT_HIP(b, hipGetLastError());
   return FLB_OK;
@@ -314,7 +315,7 @@ int flb_batch_create(const flb_instance* in, uint32_t count, int32_t storage, in
 
 int flb_batch_configure(flb_batch* b, int32_t block_x, int32_t grid, int32_t graph_chunk) {
   if (!b) return FLB_ERR_ARGUMENT;
-  if (block_x % 64 || block_x < 64 || block_x > 1024 || grid < 1 || graph_chunk < 0 ||
+  if (block_x % 64 || block_x < 64 || block_x > 1024 || grid < 0 || graph_chunk < 0 ||
       graph_chunk % 2)
     return FLB_ERR_ARGUMENT;
   drop_graph(b);
@@ -372,8 +373,9 @@ int flb_batch_step_timed(flb_batch* b, uint64_t nsteps, float* ms) {
 int flb_batch_autotune(flb_batch* b, uint32_t steps, int32_t* best_block, int32_t* best_grid,
                        float* best_ms) {
   if (!b || steps == 0) return FLB_ERR_ARGUMENT;
+  // Blocks per CU of the grid-stride walk; 0 launches one block per tile.
   const int blocks[4] = {64, 128, 256, 512};
-  const int per_cu[4] = {2, 4, 8, 16};
+  const int per_cu[4] = {4, 16, 64, 0};
   const int chunk = b->graph_chunk;
   float best = 1e30f;
   int bb = b->block_x, bg = b->grid;

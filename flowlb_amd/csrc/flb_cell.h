@@ -71,8 +71,11 @@ __device__ __forceinline__ Cell locate(const Params& p) {
 // is one row segment that starts at a multiple of 64), so its tile word is a
 // scalar load.
 // Returns the work bits of the chunk (bit kWorkSC: stream-collide, bit
-// kWorkFS1: pass 1), 0 for a wavefront past the row's end.
-__device__ __forceinline__ uint32_t chunk_work(const Params& p, const Cell& c, bool with_pub) {
+// kWorkFS1: pass 1, kChunkBody: the tile is a body of fluid, Params::tile_body),
+// 0 for a wavefront past the row's end.
+constexpr uint32_t kChunkBody = 1u << 30;
+__device__ __forceinline__ uint32_t chunk_work(const Params& p, const Cell& c, bool with_pub,
+                                               bool with_body) {
   const uint32_t tx = uint32_t(__builtin_amdgcn_readfirstlane(c.x)) / kFsTileW;
   const uint32_t y = uint32_t(__builtin_amdgcn_readfirstlane(c.y));
   if (tx >= p.work_x) return 0u;
@@ -80,7 +83,8 @@ __device__ __forceinline__ uint32_t chunk_work(const Params& p, const Cell& c, b
   const int r = int(y % kFsTileH);
   const uint32_t w = p.tile_work[i];
   const uint32_t sc = (w | (with_pub ? p.tile_pub[i] : 0u)) >> (kWorkSC + r);
-  return (sc & 1u) << kWorkSC | ((w >> (kWorkFS1 + r)) & 1u) << kWorkFS1;
+  const uint32_t body = with_body && p.tile_body[i] ? kChunkBody : 0u;
+  return (sc & 1u) << kWorkSC | ((w >> (kWorkFS1 + r)) & 1u) << kWorkFS1 | body;
 }
 
 // Periodic neighbour indices (src/cuda/cellCalculations.cuh:82-109) without
@@ -206,8 +210,13 @@ __device__ __forceinline__ void store_outgoing(const Params& p, const uint32_t* 
 // interface, inlet or outlet cell -- and the neighbour flags need not be
 // loaded.  Compiling that case separately keeps the kernel that runs the
 // bodies of fluid to fewer registers.
+// body (DENSE, uniform over the wavefront): the cell's tile and its halo are
+// plain fluid (Params::tile_body) and the update kernel will not visit it, so
+// the cell also does the update's body-of-fluid step: mass = rho, its excess
+// (the deviation) shared by eight receivers into excess_out, flag kept.
 template <class S, int ODD, bool FS, bool MOM, bool TRACE, bool DENSE = false>
-__device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell c) {
+__device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell c,
+                                                    bool body = false) {
   using R = typename S::R;
   R* const rho_f = fld<R>(p.rho);
   R* const u_f = fld<R>(p.u);
@@ -320,7 +329,13 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     for (int i = 0; i < 9; i++) sg += double(g[i]);
     m += (1.0 + sg) - double(rho);
     if (kRounded) m += double(resid);
-    fld<R>(p.mass)[n] = R(m);
+    if (DENSE && body) {
+      fld<R>(p.mass)[n] = rho;
+      fld<R>(p.excess_out)[n] = R(m) / R(8.0f);
+      p.flags_next[n] = fl;
+    } else {
+      fld<R>(p.mass)[n] = R(m);
+    }
   }
   if (iface) {  // interface -> fluid / gas test (:353-367)
     bool not_fluid = true, not_gas = true;

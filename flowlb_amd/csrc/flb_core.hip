@@ -75,7 +75,8 @@ flb::Params params(const flb_handle* h, int bx) {
   p.pub_u[0] = h->pub_u[0];
   p.pub_u[1] = h->pub_u[1];
   p.mass = h->mass;
-  p.excess = h->excess;
+  p.excess = h->excess[h->cur];
+  p.excess_out = h->excess[h->cur ^ 1];
   p.phi = h->phi;
   p.rec = h->rec;
   p.alpha = h->alpha;
@@ -85,6 +86,7 @@ flb::Params params(const flb_handle* h, int bx) {
   p.work_x = h->work_x;
   p.tile_rows = h->tile_rows;
   p.tile_list = h->tile_list;
+  p.tile_body = h->tile_body;
   return p;
 }
 
@@ -181,7 +183,7 @@ int reset_work(flb_handle* h) {
   const size_t tiles = size_t(h->work_x) * h->tile_rows;
   const size_t chunks = size_t(h->work_x) * h->ny;
   const int rd = int(h->t & 1u);
-  std::vector<uint32_t> ids(chunks), all(tiles, 0xffffffffu);
+  std::vector<uint32_t> ids(chunks), all(tiles, flb::kWorkReset);
   for (size_t i = 0; i < ids.size(); ++i) ids[i] = uint32_t(i);
   uint32_t counts[8] = {};
   counts[4 * rd + flb::kListFS1] = uint32_t(chunks);
@@ -388,7 +390,8 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   }
   if (fs) {
     if ((rc = field_alloc(h, &h->mass, Nd))) return bail(rc);
-    if ((rc = field_alloc(h, &h->excess, Nd))) return bail(rc);
+    if ((rc = field_alloc(h, &h->excess[0], Nd))) return bail(rc);
+    if ((rc = field_alloc(h, &h->excess[1], Nd))) return bail(rc);
     if ((rc = field_alloc(h, &h->phi, Nd))) return bail(rc);
     if ((rc = field_alloc(h, &h->rec, 9 * Nd))) return bail(rc);
     // Work words, lists and stamps of the free-surface kernels.
@@ -399,6 +402,7 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
     if ((rc = dev_alloc(h, &h->tile_work, tiles))) return bail(rc);
     if ((rc = dev_alloc(h, &h->tile_pub, tiles))) return bail(rc);
     if ((rc = dev_alloc(h, &h->tile_list, tiles))) return bail(rc);
+    if ((rc = dev_alloc(h, &h->tile_body, tiles))) return bail(rc);
     if ((rc = dev_alloc(h, &h->tile_counts, 2))) return bail(rc);
     if ((rc = dev_alloc(h, &h->counts, 8))) return bail(rc);
     for (int k = 0; k < 2; ++k)
@@ -478,8 +482,10 @@ int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const 
   if ((rc = upload_field(h, h->phi, phi, 1))) return rc;
   if ((rc = reset_work(h))) return rc;  // new masses: evaluate everything next step
   if ((rc = zero_field(h, h->rec, h->Nd))) return rc;  // stream-collide's mass residuals
-  if (excess) return upload_field(h, h->excess, excess, 1);
-  return zero_field(h, h->excess, h->Nd);
+  // Shares pulled by the next step; the other buffer receives the next shares.
+  if ((rc = zero_field(h, h->excess[h->cur ^ 1], h->Nd))) return rc;
+  if (excess) return upload_field(h, h->excess[h->cur], excess, 1);
+  return zero_field(h, h->excess[h->cur], h->Nd);
 }
 
 // One lattice step on the handle's stream.  ev (optional) receives per-kernel
@@ -528,7 +534,7 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
     if ((rc = run(FLB_K_FS_UPDATE, flb::get_fs_update(st, odd, h->nx, h->ny), q,
                   flb::kFsTileW * flb::kFsTileH)))
       return rc;
-    h->cur ^= 1;  // the update wrote the other flag buffer
+    h->cur ^= 1;  // the update wrote the other flag and share buffers
   } else {
     if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, false, moments), p, bx)))
       return rc;
@@ -640,7 +646,7 @@ int read_fs_fields(flb_handle* h, T* mass, T* excess) {
   if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "not a free-surface handle");
   int rc;
   if (mass && (rc = download_field(h, mass, h->mass, 1))) return rc;
-  if (excess && (rc = download_field(h, excess, h->excess, 1))) return rc;
+  if (excess && (rc = download_field(h, excess, h->excess[h->cur], 1))) return rc;
   return FLB_OK;
 }
 

@@ -120,7 +120,11 @@ struct Params {
   float* pub_rho[2];     // densities published for inlet cells, by parity
   float* pub_u[2];       // velocities published for outlet cells, by parity
   float* mass;
+  // Excess shares, double-buffered by step: excess holds the shares pulled this
+  // step (by pass 1 and stream-collide), excess_out receives the shares the
+  // update -- and stream-collide for bodies of fluid -- hand out for the next.
   float* excess;
+  float* excess_out;
   float* phi;
   float* rec;            // reconstructed gas populations, 9 planes of N
   const float* alpha;    // Brinkman solid fraction or nullptr
@@ -146,6 +150,9 @@ struct Params {
   uint32_t* tile_list;
   uint32_t* tile_count;
   uint32_t* tile_count_next;
+  // Per tile, written by pass 1: the tile and its three-cell halo are plain
+  // fluid, so stream-collide also does the update's work for its cells.
+  uint32_t* tile_body;
 };
 
 enum : int { kListFS1 = 0 };
@@ -158,6 +165,8 @@ constexpr int kUpdateBlocksPerCU = 16;
 
 constexpr int kWorkSC = 0, kWorkFS1 = 8;  // bit offsets in a tile_work word
 constexpr uint32_t kWorkQuiet = 1u << 16;  // the update found the tile quiet
+constexpr uint32_t kWorkFluid = 1u << 17;  // every cell of the tile is plain fluid
+constexpr uint32_t kWorkReset = 0xffffu;   // every chunk has work, nothing known
 constexpr uint32_t kWorkSCMask = 0xffu << kWorkSC;
 
 // A cell field viewed in the arithmetic type R.

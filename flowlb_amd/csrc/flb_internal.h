@@ -49,7 +49,7 @@ struct flb_handle {
   float* pub_rho[2] = {nullptr, nullptr};
   float* pub_u[2] = {nullptr, nullptr};
   float* mass = nullptr;
-  float* excess = nullptr;
+  float* excess[2] = {nullptr, nullptr};  // shares by step parity (index cur: read)
   float* phi = nullptr;
   float* rec = nullptr;
   uint32_t* tile_work = nullptr;  // free surface: per-tile work words (Params::tile_work)
@@ -60,6 +60,7 @@ struct flb_handle {
   uint32_t* fs1_list[2] = {nullptr, nullptr};
   uint32_t* counts = nullptr;
   uint32_t* tile_list = nullptr;  // tiles the update of this step evaluates
+  uint32_t* tile_body = nullptr;  // tiles stream-collide updates (bodies of fluid)
   uint32_t* tile_counts = nullptr;  // its length, by step parity
   int cus = 256;  // compute units of the device (grid size of the list-driven pass 1)
   float* alpha = nullptr;     // Brinkman solid fraction (allocated on first use)

@@ -32,12 +32,13 @@ namespace flb {
 template <class S, int ODD, bool FS, bool MOM, bool TRACE = false, bool DENSE = false>
 __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
   const Cell c = locate(p);
+  uint32_t work = 0;
   if (FS) {
-    const uint32_t work = chunk_work(p, c, true);
+    work = chunk_work(p, c, true, DENSE);
     if (!(work & (1u << kWorkSC))) return;
     if (DENSE && (work & (1u << kWorkFS1))) return;
   }
-  if (c.valid) stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE>(p, c);
+  if (c.valid) stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE>(p, c, (work & kChunkBody) != 0);
 }
 
 // ------------------------------------------------------------- free surface
@@ -240,9 +241,10 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
     // Empty gas cell.  A cell that turned gas in the previous step left an
     // excess share for its neighbours and marked its mass -0.0; clear both.
     p.flags_next[n] = fl;
-    if (signbit(m)) {
+    if (signbit(m)) {  // both share buffers: this step's readers are done
       mass[n] = R(0);
       fld<R>(p.excess)[n] = R(0);
+      fld<R>(p.excess_out)[n] = R(0);
     }
     return fl;
   }
@@ -348,7 +350,7 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
     ex = R(0);
   }
   mass[n] = m;
-  fld<R>(p.excess)[n] = ex;
+  fld<R>(p.excess_out)[n] = ex;
   if (write_phi) fld<R>(p.phi)[n] = ph;
   p.flags_next[n] = nfl;
   return nfl;
@@ -364,15 +366,16 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
 // pass-1 work to the next step's pass-1 list.  Block-wide: every thread of
 // the block calls it.
 __device__ __forceinline__ void finish_tile(const Params& p, int tid, uint32_t tx, uint32_t ty,
-                                            uint32_t old, bool sc, bool fs1, uint32_t* bits,
-                                            uint32_t quiet) {
-  const bool any_sc = __any(sc), any_fs1 = __any(fs1);
+                                            uint32_t old, bool sc, bool fs1, bool nonfluid,
+                                            uint32_t* bits, uint32_t quiet) {
+  const bool any_sc = __any(sc), any_fs1 = __any(fs1), any_nf = __any(nonfluid);
   const int r = tid / 64;
-  if ((tid & 63) == 0 && (any_sc || any_fs1))
-    atomicOr(bits, (any_sc ? 1u << (kWorkSC + r) : 0u) | (any_fs1 ? 1u << (kWorkFS1 + r) : 0u));
+  if ((tid & 63) == 0 && (any_sc || any_fs1 || any_nf))
+    atomicOr(bits, (any_sc ? 1u << (kWorkSC + r) : 0u) | (any_fs1 ? 1u << (kWorkFS1 + r) : 0u) |
+                       (any_nf ? kWorkFluid : 0u));
   __syncthreads();
   if (tid == 0) {
-    const uint32_t w = *bits | quiet;
+    const uint32_t w = (*bits ^ kWorkFluid) | quiet;  // the fluid bit: no wavefront saw another cell
     if (w != old) p.tile_work[ty * p.work_x + tx] = w;
     const uint32_t rows_fs1 = (w >> kWorkFS1) & 0xffu;
     if (rows_fs1) {
@@ -445,23 +448,24 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
     if (valid) {
       const R ex = cons ? m : m - rho_pre;
       mass[n] = rho_pre;
-      fld<R>(p.excess)[n] = ex / R(8.0f);
+      fld<R>(p.excess_out)[n] = ex / R(8.0f);
       p.flags_next[n] = own;
     }
-    finish_tile(p, tid, tx, ty, old_work, valid, valid && !cons, work_bits, 0u);
+    finish_tile(p, tid, tx, ty, old_work, valid, valid && !cons, false, work_bits, 0u);
     return;
   }
   if (quiet) {
     if (valid) {
       p.flags_next[n] = own;
       if (!(own & (kWall | kMovingWall | kInlet)) && (own & (kFGI | kGasInterface)) == kGas &&
-          signbit(m)) {
+          signbit(m)) {  // both share buffers: this step's readers are done
         mass[n] = R(0);
         fld<R>(p.excess)[n] = R(0);
+        fld<R>(p.excess_out)[n] = R(0);
       }
     }
     const bool sc = valid && !(own & (kGas | kWall));
-    finish_tile(p, tid, tx, ty, old_work, sc, sc, work_bits, kWorkQuiet);
+    finish_tile(p, tid, tx, ty, old_work, sc, sc, valid && own != kFluid, work_bits, kWorkQuiet);
     return;
   }
   for (int i = tid; i < (kLdsH - 2) * (kLdsW - 2); i += kUpdateThreads) {  // pass 2
@@ -493,15 +497,16 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
     f2s[j] = o;
   }
   __syncthreads();
-  bool sc = false, fs1 = false;
+  bool sc = false, fs1 = false, nonfluid = false;
   if (valid) {
     const int j = (tid / kTileW + kHalo) * kLdsW + (tid & (kTileW - 1)) + kHalo;
     bool interior;
     const uint8_t fin = update_cell<S, ODD>(p, f2s, j, f0s[j], m, rho_pre, n, x, y, interior);
     sc = !(fin & (kGas | kWall));
     fs1 = sc && !(interior && (fin & (kFGI | kInlet | kOutlet)) == kFluid);
+    nonfluid = fin != kFluid;
   }
-  finish_tile(p, tid, tx, ty, old_work, sc, fs1, work_bits, 0u);
+  finish_tile(p, tid, tx, ty, old_work, sc, fs1, nonfluid, work_bits, 0u);
 }
 
 // The update kernel walks the tile list pass 1 built for this step.  A tile
@@ -562,18 +567,27 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
     if (t < tiles) {
       const uint32_t ty = t / p.work_x, tx = t - ty * p.work_x;
       const int x0 = int(tx) * kTileW, y0 = int(ty) * kTileH;
-      bool skip = (p.tile_work[t] & kWorkQuiet) && !p.tile_pub[t] && x0 >= kHalo &&
-                  x0 + kTileW + kHalo <= int(p.nx) && y0 >= kHalo &&
-                  y0 + kTileH + kHalo <= int(p.ny);
-      if (skip) {
-        uint32_t around = 0;
+      bool skip = false, body = false;
+      if (!p.tile_pub[t] && x0 >= kHalo && x0 + kTileW + kHalo <= int(p.nx) && y0 >= kHalo &&
+          y0 + kTileH + kHalo <= int(p.ny)) {
+        uint32_t any = 0, all = ~0u;
 #pragma unroll
         for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
-          for (int dx = -1; dx <= 1; dx++) around |= p.tile_work[t + dy * int(p.work_x) + dx];
-        skip = !(around & kWorkSCMask);
+          for (int dx = -1; dx <= 1; dx++) {
+            const uint32_t w = p.tile_work[t + dy * int(p.work_x) + dx];
+            any |= w;
+            all &= w;
+          }
+        skip = (p.tile_work[t] & kWorkQuiet) && !(any & kWorkSCMask);
+        // Conservative mode: the tile and its halo are plain fluid and stay
+        // so through stream-collide, which then hands out the cells' excess
+        // and sets their mass, as the update's body-of-fluid path would.
+        body = p.mass_mode != 0 && (all & kWorkFluid) &&
+               !(p.tile_work[t] & (0xffu << kWorkFS1));
       }
-      need = !skip;
+      p.tile_body[t] = body;
+      need = !skip && !body;
     }
     // One atomic per wavefront appends its listed tiles.
     const uint64_t m = __ballot(need);

@@ -240,8 +240,8 @@ typedef struct flb_batch flb_batch;
 
 int flb_batch_create(const flb_instance* instances, uint32_t count, int32_t storage,
                      int32_t device, flb_batch** out);
-/* Threads per block (multiple of 64), blocks of the grid-stride launch, steps
-   per captured graph (even; 0 = plain launches). */
+/* Threads per block (multiple of 64), blocks of the grid-stride launch (0 =
+   one block per tile), steps per captured graph (even; 0 = plain launches). */
 int flb_batch_configure(flb_batch* b, int32_t block_x, int32_t grid, int32_t graph_chunk);
 /* Times block sizes x grid sizes over `steps` steps each (the state advances)
    and keeps the fastest. */
