@@ -40,13 +40,16 @@ def active_counts(solver):
 def algorithmic_bytes(active, inactive, s, free_surface):
     """Per-launch bytes of stream-collide and per-step bytes of the whole step.
     Stream-collide: 18 s (nine loads, nine stores) + 1 flag byte per active cell,
-    + 12 in (conservative) free-surface runs: the density store read by the
+    + 17 in (conservative) free-surface runs: the density store read by the
     later passes, the excess shares pulled from the neighbours (each share is
-    read by 8 neighbours, 4 bytes per cell once) and the mass deviation store;
-    the interface cells' extra velocity store and reconstructed populations
-    (< 0.2 % of the cells) are left out; 1 flag byte per inactive cell.
-    Free-surface step: 18 s + 38 per active cell."""
-    sc = active * (18 * s + 1 + (12 if free_surface else 0)) + inactive
+    read by 8 neighbours, 4 bytes per cell once), the mass store, and for
+    bodies of fluid -- whose update stream-collide does -- the new excess share
+    and flag (counted for every active cell; the surface cells, a few per cent
+    of them, do not write these 5 bytes); the interface cells' extra velocity
+    store and reconstructed populations (< 0.2 % of the cells) are left out;
+    1 flag byte per inactive cell.  Free-surface step: 18 s + 38 per active
+    cell."""
+    sc = active * (18 * s + 1 + (17 if free_surface else 0)) + inactive
     step = active * (18 * s + (38 if free_surface else 1)) + inactive
     return sc, step
 
