@@ -40,16 +40,15 @@ def active_counts(solver):
 def algorithmic_bytes(active, inactive, s, free_surface):
     """Per-launch bytes of stream-collide and per-step bytes of the whole step.
     Stream-collide: 18 s (nine loads, nine stores) + 1 flag byte per active cell,
-    + 17 in (conservative) free-surface runs: the density store read by the
-    later passes, the excess shares pulled from the neighbours (each share is
-    read by 8 neighbours, 4 bytes per cell once), the mass store, and for
-    bodies of fluid -- whose update stream-collide does -- the new excess share
-    and flag (counted for every active cell; the surface cells, a few per cent
-    of them, do not write these 5 bytes); the interface cells' extra velocity
-    store and reconstructed populations (< 0.2 % of the cells) are left out;
-    1 flag byte per inactive cell.  Free-surface step: 18 s + 38 per active
-    cell."""
-    sc = active * (18 * s + 1 + (17 if free_surface else 0)) + inactive
+    + 13 in (conservative) free-surface runs, as a body-of-fluid cell -- whose
+    update stream-collide does -- moves them: the excess shares pulled from
+    the neighbours (each share is read by 8 neighbours, 4 bytes per cell
+    once), the mass store, its new excess share and flag (a surface cell
+    stores its density instead of the share and flag: 1 byte less); the
+    interface cells' extra velocity store and reconstructed populations
+    (< 0.2 % of the cells) are left out; 1 flag byte per inactive cell.
+    Free-surface step: 18 s + 38 per active cell."""
+    sc = active * (18 * s + 1 + (13 if free_surface else 0)) + inactive
     step = active * (18 * s + (38 if free_surface else 1)) + inactive
     return sc, step
 

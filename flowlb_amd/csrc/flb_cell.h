@@ -372,9 +372,10 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     // of a fluid cell only at readback -- pass 1 reads the velocity of
     // inlet/outlet interface cells, pass 3 averages interface neighbours
     // (a fluid cell never neighbours a gas or gas->interface cell).
+    // The density of a body-of-fluid cell is read only at readback: the cell
+    // sets its mass itself, and no gas cell borders it.
     const bool want_u = MOM || (!DENSE && (fl & kFGI) != kFluid);
     if (!io) {
-      rho_f[n] = rho;
       if (want_u) {
         u_f[n] = ux;
         u_f[N + n] = uy;
@@ -383,7 +384,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
       u_f[n] = ux;
       u_f[N + n] = uy;
     }
-    rho_f[n] = rho;
+    if (MOM || !(DENSE && body)) rho_f[n] = rho;
   }
   if (pub_r) prho0[n] = rho;
   if (pub_v) {
