@@ -210,13 +210,15 @@ __device__ __forceinline__ void store_outgoing(const Params& p, const uint32_t* 
 // interface, inlet or outlet cell -- and the neighbour flags need not be
 // loaded.  Compiling that case separately keeps the kernel that runs the
 // bodies of fluid to fewer registers.
-// body (DENSE, uniform over the wavefront): the cell's tile and its halo are
-// plain fluid (Params::tile_body) and the update kernel will not visit it, so
-// the cell also does the update's body-of-fluid step: mass = rho, its excess
-// (the deviation) shared by eight receivers into excess_out, flag kept.
-template <class S, int ODD, bool FS, bool MOM, bool TRACE, bool DENSE = false>
-__device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell c,
-                                                    bool body = false) {
+// BODY (with DENSE): the cell's tile and its halo are plain fluid
+// (Params::tile_body) and the update kernel will not visit it, so the cell
+// also does the update's body-of-fluid step: mass = rho, its excess (the
+// deviation) shared by eight receivers into excess_out, flag kept.  Its flag
+// is known (plain fluid, no inlet/outlet neighbour role: such tiles are never
+// bodies), so the wavefront goes straight to the population loads.
+template <class S, int ODD, bool FS, bool MOM, bool TRACE, bool DENSE = false, bool BODY = false>
+__device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell c) {
+  constexpr bool body = DENSE && BODY;
   using R = typename S::R;
   R* const rho_f = fld<R>(p.rho);
   R* const u_f = fld<R>(p.u);
@@ -232,11 +234,11 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   // itself) so that they share one memory round trip instead of three
   // branch-guarded ones.
   const Linear lin = linear_neighbours(p, c.x, c.y, n);
-  const uint8_t fl = p.flags[n];
-  const uint8_t fl_prev = p.flags[lin.prev];
-  const uint8_t fl_next = p.flags[lin.next];
-  const bool pub_r = lin.has_prev && (fl_prev & kInlet);
-  const bool pub_v = lin.has_next && (fl_next & kOutlet) && !(fl & kInlet);
+  const uint8_t fl = body ? uint8_t(kFluid) : p.flags[n];
+  const uint8_t fl_prev = body ? uint8_t(kFluid) : p.flags[lin.prev];
+  const uint8_t fl_next = body ? uint8_t(kFluid) : p.flags[lin.next];
+  const bool pub_r = !body && lin.has_prev && (fl_prev & kInlet);
+  const bool pub_v = !body && lin.has_next && (fl_next & kOutlet) && !(fl & kInlet);
   if (fl & (kGas | kWall)) {
     if (FS) {  // a source cell that is gas this step keeps its last value
       if (pub_r) prho0[n] = prho1[n];
@@ -329,7 +331,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     for (int i = 0; i < 9; i++) sg += double(g[i]);
     m += (1.0 + sg) - double(rho);
     if (kRounded) m += double(resid);
-    if (DENSE && body) {
+    if (body) {
       fld<R>(p.mass)[n] = rho;
       fld<R>(p.excess_out)[n] = R(m) / R(8.0f);
       p.flags_next[n] = fl;
@@ -384,7 +386,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
       u_f[n] = ux;
       u_f[N + n] = uy;
     }
-    if (MOM || !(DENSE && body)) rho_f[n] = rho;
+    if (MOM || !body) rho_f[n] = rho;
   }
   if (pub_r) prho0[n] = rho;
   if (pub_v) {

@@ -38,7 +38,11 @@ __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
     if (!(work & (1u << kWorkSC))) return;
     if (DENSE && (work & (1u << kWorkFS1))) return;
   }
-  if (c.valid) stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE>(p, c, (work & kChunkBody) != 0);
+  if (!c.valid) return;
+  if (DENSE && (work & kChunkBody))
+    stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE, true>(p, c);
+  else
+    stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE>(p, c);
 }
 
 // ------------------------------------------------------------- free surface
