@@ -514,7 +514,7 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
     p.tile_count = h->tile_counts + rd;
     p.tile_count_next = h->tile_counts + wr;
     flb::Params q = p;  // pass 1: a grid of resident blocks walks the list
-    q.nblocks = uint32_t(2 * h->cus);
+    q.nblocks = uint32_t(flb::kListBlocksPerCU * h->cus);
     if ((rc = run(FLB_K_FS1, flb::get_fs1(st, odd), q, flb::kListThreads))) return rc;
     if (h->opt.mass_mode != FLB_MASS_CONSERVATIVE) {
       if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, moments), p, bx)))
@@ -523,7 +523,9 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
       // Bodies of fluid in one launch over the grid, the surface chunks (the
       // pass-1 list) in a second one.
       if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE], h->stream));
-      if ((rc = launch(h, flb::get_stream_collide_dense(st, odd, moments), p, bx))) return rc;
+      flb::Params d = p;
+      d.nblocks = p.tiles_x * ((h->ny + flb::kDenseRows - 1) / flb::kDenseRows);
+      if ((rc = launch(h, flb::get_stream_collide_dense(st, odd, moments), d, bx))) return rc;
       if ((rc = launch(h, flb::get_stream_collide_surface(st, odd, moments), q, flb::kListThreads)))
         return rc;
       if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE + 1], h->stream));

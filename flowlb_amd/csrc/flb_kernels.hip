@@ -31,6 +31,25 @@ namespace flb {
 // k_stream_collide_surface the others.
 template <class S, int ODD, bool FS, bool MOM, bool TRACE = false, bool DENSE = false>
 __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
+  if (DENSE && kDenseRows > 1) {
+    // kDenseRows rows of one tile per wavefront, one after the other.
+    uint32_t tx, vy;
+    block_tile(blockIdx.x, p.tiles_x, (p.ny + kDenseRows - 1) / kDenseRows, tx, vy);
+    const uint32_t x = tx * blockDim.x + threadIdx.x;
+#pragma unroll 1
+    for (uint32_t r = 0; r < kDenseRows; r++) {
+      const uint32_t y = vy * kDenseRows + r;
+      if (y >= p.ny) break;
+      const Cell c{x, y, x + y * p.pitch, x < p.nx};
+      const uint32_t work = chunk_work(p, c, true, true);
+      if (!(work & (1u << kWorkSC)) || (work & (1u << kWorkFS1)) || !c.valid) continue;
+      if (work & kChunkBody)
+        stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE, true>(p, c);
+      else
+        stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE>(p, c);
+    }
+    return;
+  }
   const Cell c = locate(p);
   uint32_t work = 0;
   if (FS) {
@@ -55,14 +74,14 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
   R* const phi = fld<R>(p.phi);
   const uint32_t n = c.n;
   const uint64_t N = p.N;
-  const uint8_t fl = p.flags[n];
-  // Most wavefronts of a free-surface domain are all gas: they leave after one
-  // byte load, before any neighbour is touched.
-  if (fl & (kGas | kWall | kInlet)) return;
+  // The own and the eight neighbour flags are loaded together (one memory
+  // round trip); gas, wall and inlet cells then leave.
   uint32_t nb[9];
   neighbours(p, c.x, c.y, nb);
+  const uint8_t fl = p.flags[n];
   uint8_t nf[9];
   neighbour_flags(p, nb, nf);
+  if (fl & (kGas | kWall | kInlet)) return;
   const bool cons = p.mass_mode != 0;
   // Conservative mode: the mass of a fluid cell whose eight neighbours are all
   // fluid or interface is updated by stream-collide instead (interior_fluid).
@@ -167,6 +186,14 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
   mass[n] = m;
 }
 
+
+// First entry a list walker takes, loaded together with the list length (the
+// index is kept inside the list's capacity, so the load is in bounds whatever
+// the length; the value is used only if i < length): the walker then waits for
+// memory once before its first item instead of twice.
+__device__ __forceinline__ uint32_t first_entry(const uint32_t* list, uint32_t i, uint32_t cap) {
+  return list[i < cap ? i : cap - 1u];
+}
 
 // Flag transition a cell applies to itself in pass 4.
 __device__ __forceinline__ uint8_t settle(uint8_t fl) {
@@ -528,9 +555,9 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   __shared__ uint32_t work_bits;
   __shared__ uint32_t busy_waves[2];  // one byte per wavefront
   static_assert(kUpdateThreads / 64 <= 8, "one byte per wavefront");
-  const uint32_t count = *p.tile_count;
   uint32_t i = blockIdx.x;
-  uint32_t tile = i < count ? p.tile_list[i] : 0u;
+  uint32_t tile = first_entry(p.tile_list, i, p.work_x * p.tile_rows);
+  const uint32_t count = *p.tile_count;
   while (i < count) {
     const uint32_t j = i + gridDim.x;
     const uint32_t next = j < count ? p.tile_list[j] : 0u;
@@ -546,49 +573,69 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
 // update kernel of this step fills and the counter of the next step's tile
 // list (their last readers, the kernels of the previous step, have finished),
 // and lists the tiles the update of this step must evaluate: all but those
-// quiet in their last update (kWorkQuiet), not on the domain edge -- where the
-// three-cell halo may reach past the neighbour tiles -- and with only gas and
-// walls in the 3 x 3 tiles around them after the last step.
+// quiet in their last update (kWorkQuiet) with only gas and walls in the 3 x 3
+// tiles around them (periodic, as the lattice) after the last step.
 template <class S, int ODD>
 __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
   if (blockIdx.x == 0 && threadIdx.x == 0) p.counts_out[kListFS1] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 1) *p.tile_count_next = 0;
-  const uint32_t count = p.counts_in[kListFS1];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nw = gridDim.x * (blockDim.x / 64u);
-  for (uint32_t i = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; i < count; i += nw) {
-    const uint32_t c = p.fs1_list[i];
+  uint32_t i = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u;
+  uint32_t c = first_entry(p.fs1_list, i, p.work_x * p.ny);
+  const uint32_t count = p.counts_in[kListFS1];
+  for (; i < count; i += nw) {
+    const uint32_t j = i + nw;
+    const uint32_t next = j < count ? p.fs1_list[j] : 0u;
     const uint32_t y = c / p.work_x, x = (c - y * p.work_x) * kFsTileW + lane;
     const Cell cell{x, y, x + y * p.pitch, x < p.nx};
     if (cell.valid) fs1_cell<S, ODD>(p, cell);
+    c = next;
   }
   const uint32_t tiles = p.work_x * p.tile_rows;
   const uint32_t nt = gridDim.x * blockDim.x;
-  // One tile per lane; the loop is uniform over the wavefront (ballots).
+  // A tile's three-cell halo lies in the 3 x 3 tiles around it, taken
+  // periodically as the lattice is, unless a last tile column (row) of fewer
+  // than three cells lets it spill one tile further: then the first tile
+  // column (row) and the one before the last are never skipped.
+  const bool thin_x = p.nx - (p.work_x - 1u) * kTileW < uint32_t(kHalo) && p.work_x >= 4u;
+  const bool thin_y = p.ny - (p.tile_rows - 1u) * kTileH < uint32_t(kHalo) && p.tile_rows >= 4u;
+  // One tile per lane; the loop is uniform over the wavefront (ballots).  Each
+  // lane loads the publisher word and the 3 x 3 work words at once, so the
+  // classification waits for memory once.
   for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < tiles; t0 += nt) {
     const uint32_t t = t0 + lane;
     bool need = false;
     if (t < tiles) {
       const uint32_t ty = t / p.work_x, tx = t - ty * p.work_x;
-      const int x0 = int(tx) * kTileW, y0 = int(ty) * kTileH;
+      const uint32_t xm = tx > 0 ? tx - 1 : p.work_x - 1, xp = tx + 1 < p.work_x ? tx + 1 : 0;
+      const uint32_t ym = ty > 0 ? ty - 1 : p.tile_rows - 1;
+      const uint32_t yp = ty + 1 < p.tile_rows ? ty + 1 : 0;
+      const uint32_t pub = p.tile_pub[t];
+      uint32_t w[9];
+      const uint32_t rows[3] = {ym * p.work_x, ty * p.work_x, yp * p.work_x};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        w[3 * k] = p.tile_work[rows[k] + xm];
+        w[3 * k + 1] = p.tile_work[rows[k] + tx];
+        w[3 * k + 2] = p.tile_work[rows[k] + xp];
+      }
+      const bool spill = (thin_x && (tx == 0 || tx + 2 == p.work_x)) ||
+                         (thin_y && (ty == 0 || ty + 2 == p.tile_rows));
       bool skip = false, body = false;
-      if (!p.tile_pub[t] && x0 >= kHalo && x0 + kTileW + kHalo <= int(p.nx) && y0 >= kHalo &&
-          y0 + kTileH + kHalo <= int(p.ny)) {
+      if (!pub && !spill) {
         uint32_t any = 0, all = ~0u;
 #pragma unroll
-        for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-          for (int dx = -1; dx <= 1; dx++) {
-            const uint32_t w = p.tile_work[t + dy * int(p.work_x) + dx];
-            any |= w;
-            all &= w;
-          }
-        skip = (p.tile_work[t] & kWorkQuiet) && !(any & kWorkSCMask);
+        for (int k = 0; k < 9; k++) {
+          any |= w[k];
+          all &= w[k];
+        }
+        const uint32_t own = w[4];
+        skip = (own & kWorkQuiet) && !(any & kWorkSCMask);
         // Conservative mode: the tile and its halo are plain fluid and stay
         // so through stream-collide, which then hands out the cells' excess
         // and sets their mass, as the update's body-of-fluid path would.
-        body = p.mass_mode != 0 && (all & kWorkFluid) &&
-               !(p.tile_work[t] & (0xffu << kWorkFS1));
+        body = p.mass_mode != 0 && (all & kWorkFluid) && !(own & (0xffu << kWorkFS1));
       }
       p.tile_body[t] = body;
       need = !skip && !body;
@@ -607,14 +654,18 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
 // k_stream_collide<..., DENSE> runs the rest.
 template <class S, int ODD, bool MOM>
 __global__ void __launch_bounds__(kListThreads) k_stream_collide_surface(const Params p) {
-  const uint32_t count = p.counts_in[kListFS1];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nw = gridDim.x * (blockDim.x / 64u);
-  for (uint32_t i = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; i < count; i += nw) {
-    const uint32_t c = p.fs1_list[i];
+  uint32_t i = blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u;
+  uint32_t c = first_entry(p.fs1_list, i, p.work_x * p.ny);
+  const uint32_t count = p.counts_in[kListFS1];
+  for (; i < count; i += nw) {
+    const uint32_t j = i + nw;
+    const uint32_t next = j < count ? p.fs1_list[j] : 0u;
     const uint32_t y = c / p.work_x, x = (c - y * p.work_x) * kFsTileW + lane;
     const Cell cell{x, y, x + y * p.pitch, x < p.nx};
     if (cell.valid) stream_collide_cell<S, ODD, true, MOM, false>(p, cell);
+    c = next;
   }
 }
 
