@@ -99,6 +99,43 @@ def measure(case, storage, steps, warmup, profile_steps, **opt):
                 finite=bool(mass_ok))
 
 
+def hbm_reference():
+    """Rates a plain device copy and an in-place update (x += 1) of a 1 GiB FP32
+    array reach on this GPU (torch's kernels, 10 launches each, events on
+    torch's stream): what this device's HBM delivers in practice, next to the
+    published 8 TB/s peak."""
+    import torch
+    n = (1 << 30) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    out = {}
+    for key, fn in (("copy_GBs", lambda: y.copy_(x)), ("inplace_update_GBs", lambda: x.add_(1.0))):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        out[key] = round(2 * 4 * n / (ms * 1e-3) / 1e9, 1)
+    del x, y
+    torch.cuda.empty_cache()
+    return out
+
+
+def hbm_reference_child():
+    """hbm_reference() in a child process: torch brings its own HIP runtime,
+    which does not start in a process where the solver library's has."""
+    import subprocess
+    r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--workload", "hbm_reference"],
+                       capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip().splitlines()[-1] if r.stderr.strip() else "failed"}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def brinkman_window(case, seed=None):
     x0, x1, y0, y1 = case.extra["window"]
     rng = np.random.default_rng(case.seed if seed is None else seed)
@@ -280,10 +317,16 @@ def main():
 
     if args.workload == "copy_probe":
         return copy_probe(args)
+    if args.workload == "hbm_reference":
+        print(json.dumps(hbm_reference()), flush=True)
+        return
     if args.workload.startswith("adjoint"):
         return bench_adjoint(args)
     if args.workload.startswith("batch"):
         return bench_batch(args)
+    # The child runs before this process loads the solver library or touches
+    # the GPU.
+    hbm_ref = hbm_reference_child()
     from flowlb_amd import cases, core
     case = cases.CASES[args.workload]
     r = measure(case, args.storage, args.steps, args.warmup, args.profile_steps)
@@ -314,8 +357,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": "stream_collide" + (" (bodies-of-fluid + surface launches, timed "
-                                                   "as one)" if case.free_surface else ""),
+                     "kernel": "stream_collide" + (" (one launch: surface chunks, interior fluid "
+                                                   "and bodies of fluid)" if case.free_surface
+                                                   else ""),
                      "algorithmic_bytes_per_launch": int(r["sc_bytes"]),
                      "avg_launch_ms": round(r["sc_ms"], 5)},
         "mlups_active_cells": round(r["active"] / (ms_step * 1e-3) / 1e6, 1),
@@ -325,6 +369,10 @@ def main():
         "kernel_ms": {k: round(v, 5) for k, v in zip(core.KERNEL_NAMES, r["per_kernel_ms"])},
         "wall_s_timed_region": round(r["wall"], 4),
     }
+    ref = dict(hbm_ref)
+    if traffic:
+        ref["stream_collide_measured_traffic_GBs"] = round(traffic / (r["sc_ms"] * 1e-3) / 1e9, 1)
+    out["hbm_reference"] = ref
     if not args.no_sweep:
         sweep = {}
         for st in STORAGE_BYTES:

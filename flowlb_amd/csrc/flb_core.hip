@@ -519,13 +519,14 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
     if (h->opt.mass_mode != FLB_MASS_CONSERVATIVE) {
       if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, moments), p, bx)))
         return rc;
+    } else if (flb::KernelFn all = flb::get_stream_collide_all(st, odd, moments)) {
+      // Surface chunks, interior fluid and bodies of fluid in one launch.
+      if ((rc = run(FLB_K_STREAM_COLLIDE, all, p, bx))) return rc;
     } else {
-      // Bodies of fluid in one launch over the grid, the surface chunks (the
-      // pass-1 list) in a second one.
+      // FP64: bodies of fluid in one launch over the grid, the surface chunks
+      // (the pass-1 list) in a second one.
       if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE], h->stream));
-      flb::Params d = p;
-      d.nblocks = p.tiles_x * ((h->ny + flb::kDenseRows - 1) / flb::kDenseRows);
-      if ((rc = launch(h, flb::get_stream_collide_dense(st, odd, moments), d, bx))) return rc;
+      if ((rc = launch(h, flb::get_stream_collide_dense(st, odd, moments), p, bx))) return rc;
       if ((rc = launch(h, flb::get_stream_collide_surface(st, odd, moments), q, flb::kListThreads)))
         return rc;
       if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE + 1], h->stream));

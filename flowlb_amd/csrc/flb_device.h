@@ -157,10 +157,10 @@ struct Params {
 
 enum : int { kListFS1 = 0 };
 constexpr int kListThreads = 256;  // block size of the list-driven pass 1
-// Grid of pass 1 and of the surface stream-collide launch, which walk the
-// pass-1 chunk list with one wavefront per chunk.  On the 8192 x 2048 dam
-// break (about 2500 chunks) the step took 132 us with 1 block per CU, 126 us
-// with 2 and 121 us with 4 or 8.
+// Grid of pass 1 and of the surface stream-collide launch (FP64 storage),
+// which walk the pass-1 chunk list with one wavefront per chunk.  On the
+// 8192 x 2048 dam break (about 2500 chunks) the step took 132 us with 1
+// block per CU, 126 us with 2 and 121 us with 4 or 8.
 constexpr int kListBlocksPerCU = 4;
 // Grid of the update's walk over its tile list.  On the 8192 x 2048 dam break
 // the update took 47 us with 4 blocks per CU (all resident), 39 us with 16
@@ -193,9 +193,6 @@ __device__ __forceinline__ R* fld(float* p) {
 // us).  Rotating each row by y tiles was as fast at 32 tiles per row but not
 // at 33.  A whole band (32 rows) is in flight at once, as 32 rows were in
 // row-major order.
-// Rows of a tile that one wavefront of the bodies-of-fluid stream-collide
-// launch walks (1, 2, 4 or 8).
-constexpr uint32_t kDenseRows = 1;
 constexpr uint32_t kTileOrder = 2;  // 0 row-major, 1 rows rotated by y, 2 bands
 constexpr uint32_t kBandRows = 32;
 __device__ __forceinline__ void block_tile(uint32_t b, uint32_t tiles_x, uint32_t rows,

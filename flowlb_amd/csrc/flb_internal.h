@@ -19,6 +19,7 @@ using HistFn = void (*)(const Params, unsigned long long*);
 KernelFn get_stream_collide(int storage, int odd, bool fs, bool mom);
 KernelFn get_stream_collide_dense(int storage, int odd, bool mom);
 KernelFn get_stream_collide_surface(int storage, int odd, bool mom);
+KernelFn get_stream_collide_all(int storage, int odd, bool mom);  // nullptr for FP64
 KernelFn get_fs1(int storage, int odd);
 KernelFn get_fs_update(int storage, int odd, uint32_t nx, uint32_t ny);
 KernelFn get_stream_collide_trace(int odd, bool mom);

@@ -31,25 +31,6 @@ namespace flb {
 // k_stream_collide_surface the others.
 template <class S, int ODD, bool FS, bool MOM, bool TRACE = false, bool DENSE = false>
 __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
-  if (DENSE && kDenseRows > 1) {
-    // kDenseRows rows of one tile per wavefront, one after the other.
-    uint32_t tx, vy;
-    block_tile(blockIdx.x, p.tiles_x, (p.ny + kDenseRows - 1) / kDenseRows, tx, vy);
-    const uint32_t x = tx * blockDim.x + threadIdx.x;
-#pragma unroll 1
-    for (uint32_t r = 0; r < kDenseRows; r++) {
-      const uint32_t y = vy * kDenseRows + r;
-      if (y >= p.ny) break;
-      const Cell c{x, y, x + y * p.pitch, x < p.nx};
-      const uint32_t work = chunk_work(p, c, true, true);
-      if (!(work & (1u << kWorkSC)) || (work & (1u << kWorkFS1)) || !c.valid) continue;
-      if (work & kChunkBody)
-        stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE, true>(p, c);
-      else
-        stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE>(p, c);
-    }
-    return;
-  }
   const Cell c = locate(p);
   uint32_t work = 0;
   if (FS) {
@@ -62,6 +43,28 @@ __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
     stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE, true>(p, c);
   else
     stream_collide_cell<S, ODD, FS, MOM, TRACE, DENSE>(p, c);
+}
+
+// Conservative free-surface stream-collide in one launch over the grid: chunks
+// with pass-1 work (the surface) take the general path, the others the
+// low-register paths for interior fluid and bodies of fluid.  The register
+// budget is capped at 72 VGPRs (7 wavefronts per SIMD): the general path
+// spills a little, which costs less than a second launch for the surface
+// chunks (8192 x 2048 dam break: 118 -> 114.6 us per step in FP32, 107.4 ->
+// 103.1 us in FP16S; a cap of 64 VGPRs spills more and gained less).  The
+// FP64 storage needs more registers and keeps the two launches.
+template <class S, int ODD, bool MOM>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7)))
+k_stream_collide_all(const Params p) {
+  const Cell c = locate(p);
+  const uint32_t work = chunk_work(p, c, true, true);
+  if (!(work & (1u << kWorkSC)) || !c.valid) return;
+  if (work & (1u << kWorkFS1))
+    stream_collide_cell<S, ODD, true, MOM, false, false>(p, c);
+  else if (work & kChunkBody)
+    stream_collide_cell<S, ODD, true, MOM, false, true, true>(p, c);
+  else
+    stream_collide_cell<S, ODD, true, MOM, false, true>(p, c);
 }
 
 // ------------------------------------------------------------- free surface
@@ -279,9 +282,7 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
     }
     return fl;
   }
-  // Non-gas cells had their density loaded before the flag passes; a gas cell
-  // turning interface (or holding stray mass) loads it now.
-  const R rho = (f0 & (kGas | kWall | kMovingWall | kInlet)) ? fld<R>(p.rho)[n] : rho_pre;
+  const R rho = rho_pre;  // loaded before the flag passes (nothing here writes rho)
   if (e == kGasInterface) {
     // Pass 3: calculateAverageValuesNeighbors2D (cellCalculations.cuh:124-145)
     // over the fluid/interface neighbours, accumulated in float as the
@@ -434,9 +435,8 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
   const uint32_t old_work = p.tile_work[ty * p.work_x + tx];
   if (tid == 0) *work_bits = 0;  // ordered before its use by the next barrier
   // All global loads of the tile are issued before any is used: the two halo
-  // bytes of this thread, its own flag and mass.  The density of a non-gas
-  // cell depends only on the own flag, so its load follows at once and
-  // arrives while the flag passes run.
+  // bytes of this thread, its own flag, mass and density (read for every
+  // cell, though only non-gas cells and gas cells turning interface use it).
   // Loads are unconditional with selected indices (lanes without a use read
   // cell 0, one cached line): a load under a divergent branch makes the
   // compiler wait for it where the branch merges.
@@ -452,8 +452,7 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
       p.flags[wrap<SMALL>(x0 + c0 - kHalo, nx) + wrap<SMALL>(y0 + r0 - kHalo, ny) * pitch];
   const uint8_t h1 =
       p.flags[wrap<SMALL>(x0 + c1 - kHalo, nx) + wrap<SMALL>(y0 + r1 - kHalo, ny) * pitch];
-  const bool dense = valid && !(own & (kGas | kWall | kMovingWall | kInlet));
-  const R rho_pre = fld<R>(p.rho)[dense ? n : 0u];
+  const R rho_pre = fld<R>(p.rho)[n];
   f0s[i0] = h0;
   f0s[i1] = h1;
   // A flag changes in passes 2-4 only within three cells of a fluid or
@@ -789,6 +788,11 @@ KernelFn dense_fn(int odd, bool mom) {
              : k_stream_collide<S, 0, true, false, false, true>;
 }
 template <class S>
+KernelFn all_fn(int odd, bool mom) {
+  if (mom) return odd ? k_stream_collide_all<S, 1, true> : k_stream_collide_all<S, 0, true>;
+  return odd ? k_stream_collide_all<S, 1, false> : k_stream_collide_all<S, 0, false>;
+}
+template <class S>
 KernelFn surface_fn(int odd, bool mom) {
   if (mom) return odd ? k_stream_collide_surface<S, 1, true> : k_stream_collide_surface<S, 0, true>;
   return odd ? k_stream_collide_surface<S, 1, false> : k_stream_collide_surface<S, 0, false>;
@@ -829,6 +833,14 @@ KernelFn get_stream_collide_dense(int storage, int odd, bool mom) {
     case 2: return dense_fn<StoreBF16S>(odd, mom);
     case 3: return dense_fn<StoreF64>(odd, mom);
     default: return dense_fn<StoreF32>(odd, mom);
+  }
+}
+KernelFn get_stream_collide_all(int storage, int odd, bool mom) {
+  switch (storage) {
+    case 1: return all_fn<StoreF16S>(odd, mom);
+    case 2: return all_fn<StoreBF16S>(odd, mom);
+    case 3: return nullptr;  // FP64: dense + surface launches
+    default: return all_fn<StoreF32>(odd, mom);
   }
 }
 KernelFn get_stream_collide_surface(int storage, int odd, bool mom) {
