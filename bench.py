@@ -305,6 +305,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-hbm-reference", action="store_true",
+                    help="skip the copy / in-place rates measured in a child process")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--fd-cells", type=int, default=16)
     ap.add_argument("--instances", type=int, default=256)
@@ -325,8 +327,11 @@ def main():
     if args.workload.startswith("batch"):
         return bench_batch(args)
     # The child runs before this process loads the solver library or touches
-    # the GPU.
-    hbm_ref = hbm_reference_child()
+    # the GPU -- and not at all under a profiler, whose preloaded library has
+    # started the GPU runtime in this process already.
+    profiled = "rocprof" in os.environ.get("LD_PRELOAD", "") or any(
+        k.startswith("ROCPROF") for k in os.environ)
+    hbm_ref = {} if args.no_hbm_reference or profiled else hbm_reference_child()
     from flowlb_amd import cases, core
     case = cases.CASES[args.workload]
     r = measure(case, args.storage, args.steps, args.warmup, args.profile_steps)
@@ -369,10 +374,11 @@ def main():
         "kernel_ms": {k: round(v, 5) for k, v in zip(core.KERNEL_NAMES, r["per_kernel_ms"])},
         "wall_s_timed_region": round(r["wall"], 4),
     }
-    ref = dict(hbm_ref)
-    if traffic:
-        ref["stream_collide_measured_traffic_GBs"] = round(traffic / (r["sc_ms"] * 1e-3) / 1e9, 1)
-    out["hbm_reference"] = ref
+    if hbm_ref:
+        ref = dict(hbm_ref)
+        if traffic:
+            ref["stream_collide_measured_traffic_GBs"] = round(traffic / (r["sc_ms"] * 1e-3) / 1e9, 1)
+        out["hbm_reference"] = ref
     if not args.no_sweep:
         sweep = {}
         for st in STORAGE_BYTES:
