@@ -64,6 +64,15 @@ __device__ __forceinline__ Cell locate(const Params& p) {
   return Cell{x, y, x + y * p.pitch, x < p.nx};
 }
 
+// Cell of this thread in a free-surface stream-collide launch over the window
+// of Params (block_tile_window).
+__device__ __forceinline__ Cell locate_window(const Params& p) {
+  uint32_t tx, y;
+  block_tile_window(p, blockIdx.x, tx, y);
+  const uint32_t x = tx * blockDim.x + threadIdx.x;
+  return Cell{x, y, x + y * p.pitch, x < p.nx};
+}
+
 // Free surface: does the 64-cell row chunk of this wavefront have work for the
 // pass whose bits start at `base` (kWorkSC, kWorkFS1)?  In conservative mode
 // pass 1 has none in chunks of gas and interior fluid, stream-collide none in

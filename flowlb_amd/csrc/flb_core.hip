@@ -87,6 +87,12 @@ flb::Params params(const flb_handle* h, int bx) {
   p.tile_rows = h->tile_rows;
   p.tile_list = h->tile_list;
   p.tile_body = h->tile_body;
+  p.box = h->box;
+  p.box_host = h->box_host_dev;
+  p.step = h->t;
+  p.band0 = 0;
+  p.col0 = 0;
+  p.ncols = p.tiles_x;
   return p;
 }
 
@@ -192,7 +198,12 @@ int reset_work(flb_handle* h) {
   if ((rc = upload(h, h->tile_work, all.data(), tiles * sizeof(uint32_t)))) return rc;
   if ((rc = upload(h, h->fs1_list[rd], ids.data(), chunks * sizeof(uint32_t)))) return rc;
   if ((rc = upload(h, h->tile_counts, zero2, sizeof(zero2)))) return rc;
-  return upload(h, h->counts, counts, sizeof(counts));
+  if ((rc = upload(h, h->counts, counts, sizeof(counts)))) return rc;
+  // The published stream-collide box is marked unknown once the uploads
+  // (which wait for the stream) are done, so that no update still in flight
+  // can overwrite the mark.
+  *static_cast<volatile unsigned long long*>(&h->box_host[1]) = ~0ull;
+  return FLB_OK;
 }
 
 // count: device elements.
@@ -329,6 +340,9 @@ void flb_destroy(flb_handle* h) {
     (void)hipStreamDestroy(h->stream);
   }
   for (void* v : h->owned) (void)hipFree(v);
+  if (h->box_host) (void)hipHostFree(h->box_host);
+  for (hipEvent_t e : h->in_flight)
+    if (e) (void)hipEventDestroy(e);
   delete h;
 }
 
@@ -405,6 +419,46 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
     if ((rc = dev_alloc(h, &h->tile_body, tiles))) return bail(rc);
     if ((rc = dev_alloc(h, &h->tile_counts, 2))) return bail(rc);
     if ((rc = dev_alloc(h, &h->counts, 8))) return bail(rc);
+    if ((rc = dev_alloc(h, &h->box, 4 * ((tiles + 63) / 64)))) return bail(rc);
+    // A domain edge whose first or last row (column) is all wall: work cannot
+    // spread across that periodic seam (walls never change), so the
+    // stream-collide window need not wrap there.
+    {
+      bool row0 = true, row1 = true, col0 = true, col1 = true;
+      for (uint32_t x = 0; x < nx; ++x) {
+        row0 = row0 && (flags[x] & FLB_WALL);
+        row1 = row1 && (flags[uint64_t(ny - 1) * nx + x] & FLB_WALL);
+      }
+      for (uint32_t y = 0; y < ny; ++y) {
+        col0 = col0 && (flags[uint64_t(y) * nx] & FLB_WALL);
+        col1 = col1 && (flags[uint64_t(y) * nx + nx - 1] & FLB_WALL);
+      }
+      h->seam_y_closed = row0 || row1;
+      h->seam_x_closed = col0 || col1;
+    }
+    {
+      void* hb = nullptr;
+      if (hipHostMalloc(&hb, 2 * sizeof(unsigned long long),
+                        hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        (void)hipGetLastError();
+        return bail(FLB_ERR_MEMORY);
+      }
+      h->box_host = static_cast<unsigned long long*>(hb);
+      h->box_host[0] = 0ull;
+      h->box_host[1] = ~0ull;
+      void* hd = nullptr;
+      if (hipHostGetDevicePointer(&hd, hb, 0) != hipSuccess || !hd) {
+        (void)hipGetLastError();
+        return bail(FLB_ERR_DEVICE);
+      }
+      h->box_host_dev = static_cast<unsigned long long*>(hd);
+      for (hipEvent_t& e : h->in_flight)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+          (void)hipGetLastError();
+          e = nullptr;
+          return bail(FLB_ERR_DEVICE);
+        }
+    }
     for (int k = 0; k < 2; ++k)
       if ((rc = dev_alloc(h, &h->fs1_list[k], chunks))) return bail(rc);
     std::vector<uint32_t> pub(tiles, 0u);
@@ -488,6 +542,48 @@ int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const 
   return zero_field(h, h->excess[h->cur], h->Nd);
 }
 
+// Window of the free-surface stream-collide launch for step h->t.  The last
+// box the update published (step s <= t) bounds the chunks with
+// stream-collide work at step s; each step can add work only one cell beyond
+// it (a gas cell turns interface only next to an interface cell turning
+// fluid), so at step t the work lies in the box widened by t - s cells.  A
+// widened box that reaches the domain edge takes the whole extent in that
+// direction (the lattice is periodic), unless a wall row (column) closes that
+// seam; an empty box means no work at all (nblocks 0).  Without a published
+// box the window is the whole grid.
+void stream_collide_window(const flb_handle* h, flb::Params& p, int bx) {
+  if (!h->box_host || h->work_x >= 0xffffu || h->tile_rows >= 0xffffu) return;
+  const volatile unsigned long long* bh = h->box_host;
+  const unsigned long long s = bh[1];  // the step first, then the box
+  if (s == ~0ull || s > h->t) return;
+  const unsigned long long b = bh[0];
+  const int64_t ylo_t = int64_t(b & 0xffffu), yhi_t = int64_t((b >> 16) & 0xffffu);
+  const int64_t xlo_t = int64_t((b >> 32) & 0xffffu), xhi_t = int64_t((b >> 48) & 0xffffu);
+  if (ylo_t > yhi_t || xlo_t > xhi_t) {
+    p.nblocks = 0;
+    return;
+  }
+  const int64_t k = int64_t(h->t - s);
+  const int64_t ny = h->ny, nx = h->nx;
+  const int64_t ylo = ylo_t * flb::kFsTileH - k, yhi = (yhi_t + 1) * flb::kFsTileH - 1 + k;
+  const int64_t xlo = xlo_t * flb::kFsTileW - k, xhi = (xhi_t + 1) * flb::kFsTileW - 1 + k;
+  int64_t band0 = 0, band1 = (ny - 1) / flb::kBandRows;
+  if (h->seam_y_closed || (ylo >= 0 && yhi < ny)) {
+    band0 = std::max<int64_t>(ylo, 0) / flb::kBandRows;
+    band1 = std::min<int64_t>(yhi, ny - 1) / flb::kBandRows;
+  }
+  int64_t col0 = 0, col1 = int64_t(p.tiles_x) - 1;
+  if (h->seam_x_closed || (xlo >= 0 && xhi < nx)) {
+    col0 = std::max<int64_t>(xlo, 0) / bx;
+    col1 = std::min<int64_t>(std::min<int64_t>(xhi, nx - 1) / bx, col1);
+  }
+  const int64_t rows = std::min<int64_t>(ny, (band1 + 1) * flb::kBandRows) - band0 * flb::kBandRows;
+  p.band0 = uint32_t(band0);
+  p.col0 = uint32_t(col0);
+  p.ncols = uint32_t(col1 - col0 + 1);
+  p.nblocks = uint32_t(int64_t(p.ncols) * rows);
+}
+
 // One lattice step on the handle's stream.  ev (optional) receives per-kernel
 // event pairs: ev[2k], ev[2k+1] around kernel k.
 static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
@@ -520,13 +616,21 @@ static int enqueue_step(flb_handle* h, bool moments, hipEvent_t* ev) {
       if ((rc = run(FLB_K_STREAM_COLLIDE, flb::get_stream_collide(st, odd, true, moments), p, bx)))
         return rc;
     } else if (flb::KernelFn all = flb::get_stream_collide_all(st, odd, moments)) {
-      // Surface chunks, interior fluid and bodies of fluid in one launch.
-      if ((rc = run(FLB_K_STREAM_COLLIDE, all, p, bx))) return rc;
-    } else {
-      // FP64: bodies of fluid in one launch over the grid, the surface chunks
-      // (the pass-1 list) in a second one.
+      // Surface chunks, interior fluid and bodies of fluid in one launch over
+      // the window that can hold work.
+      flb::Params w = p;
+      stream_collide_window(h, w, bx);
       if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE], h->stream));
-      if ((rc = launch(h, flb::get_stream_collide_dense(st, odd, moments), p, bx))) return rc;
+      if (w.nblocks && (rc = launch(h, all, w, bx))) return rc;
+      if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE + 1], h->stream));
+    } else {
+      // FP64: bodies of fluid in one launch over the window, the surface
+      // chunks (the pass-1 list) in a second one.
+      flb::Params w = p;
+      stream_collide_window(h, w, bx);
+      if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE], h->stream));
+      if (w.nblocks && (rc = launch(h, flb::get_stream_collide_dense(st, odd, moments), w, bx)))
+        return rc;
       if ((rc = launch(h, flb::get_stream_collide_surface(st, odd, moments), q, flb::kListThreads)))
         return rc;
       if (ev) FLB_HIP(h, hipEventRecord(ev[2 * FLB_K_STREAM_COLLIDE + 1], h->stream));
@@ -571,9 +675,15 @@ extern "C" {
 
 int flb_step(flb_handle* h, uint64_t nsteps) {
   if (!h) return FLB_ERR_ARGUMENT;
+  const bool throttle = h->box_host != nullptr;
+  constexpr int K = flb_handle::kInFlight;
   for (uint64_t k = 0; k < nsteps; ++k) {
+    // Free surface: wait until the step K back has finished, so that the
+    // published stream-collide box is at most about K steps old.
+    if (throttle && k >= uint64_t(K)) FLB_HIP(h, hipEventSynchronize(h->in_flight[k % K]));
     const int rc = enqueue_step(h, k + 1 == nsteps, nullptr);
     if (rc) return rc;
+    if (throttle) FLB_HIP(h, hipEventRecord(h->in_flight[k % K], h->stream));
   }
   return FLB_OK;
 }

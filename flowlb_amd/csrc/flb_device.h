@@ -153,6 +153,17 @@ struct Params {
   // Per tile, written by pass 1: the tile and its three-cell halo are plain
   // fluid, so stream-collide also does the update's work for its cells.
   uint32_t* tile_body;
+  // Free surface: bounding box, in tiles (row lo, row hi, column lo, column
+  // hi), of the chunks with stream-collide work.  Pass 1 writes one box per
+  // 64 tiles; the update of step t reduces them and publishes the result to
+  // box_host (mapped host memory: the packed box, then t).
+  uint32_t* box;
+  unsigned long long* box_host;
+  uint64_t step;  // t
+  // Window of the free-surface stream-collide launch (locate_window): bands of
+  // kBandRows rows from band0, block columns col0 .. col0 + ncols - 1.  The
+  // whole grid is 0, 0, tiles_x.
+  uint32_t band0, col0, ncols;
 };
 
 enum : int { kListFS1 = 0 };
@@ -210,6 +221,21 @@ __device__ __forceinline__ void block_tile(uint32_t b, uint32_t tiles_x, uint32_
   y = b / tiles_x;
   tx = b - y * tiles_x;
   if (kTileOrder == 1) tx = (tx + y % tiles_x) % tiles_x;
+}
+
+// block_tile (band order) restricted to the window band0 / col0 / ncols of
+// Params: the same mapping as block_tile over the whole grid when the window
+// is (0, 0, tiles_x).
+__device__ __forceinline__ void block_tile_window(const Params& p, uint32_t b, uint32_t& tx,
+                                                  uint32_t& y) {
+  const uint32_t per_band = kBandRows * p.ncols;
+  const uint32_t band = b / per_band;
+  const uint32_t r = b - band * per_band;
+  const uint32_t y0 = (p.band0 + band) * kBandRows;
+  const uint32_t h = p.ny - y0 < kBandRows ? p.ny - y0 : kBandRows;
+  const uint32_t c = r / h;
+  tx = p.col0 + c;
+  y = y0 + (r - c * h);
 }
 
 }  // namespace flb

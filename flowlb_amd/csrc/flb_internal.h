@@ -63,6 +63,16 @@ struct flb_handle {
   uint32_t* tile_list = nullptr;  // tiles the update of this step evaluates
   uint32_t* tile_body = nullptr;  // tiles stream-collide updates (bodies of fluid)
   uint32_t* tile_counts = nullptr;  // its length, by step parity
+  // Stream-collide launch window: the box of chunks with stream-collide work
+  // (device, by step parity) and its copy in mapped host memory (box, step).
+  uint32_t* box = nullptr;
+  unsigned long long* box_host = nullptr;      // host view
+  unsigned long long* box_host_dev = nullptr;  // its device address (Params::box_host)
+  bool seam_y_closed = false, seam_x_closed = false;  // a wall row / column on the edge
+  // flb_step keeps at most kInFlight steps queued, so that the box the host
+  // sizes a launch from is at most a few steps old.
+  static constexpr int kInFlight = 8;
+  hipEvent_t in_flight[kInFlight] = {};
   int cus = 256;  // compute units of the device (grid size of the list-driven pass 1)
   float* alpha = nullptr;     // Brinkman solid fraction (allocated on first use)
   float* rho_init = nullptr;  // rho0 and u0 as given to flb_create

@@ -31,7 +31,7 @@ namespace flb {
 // k_stream_collide_surface the others.
 template <class S, int ODD, bool FS, bool MOM, bool TRACE = false, bool DENSE = false>
 __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
-  const Cell c = locate(p);
+  const Cell c = FS ? locate_window(p) : locate(p);
   uint32_t work = 0;
   if (FS) {
     work = chunk_work(p, c, true, DENSE);
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
 template <class S, int ODD, bool MOM>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7)))
 k_stream_collide_all(const Params p) {
-  const Cell c = locate(p);
+  const Cell c = locate_window(p);
   const uint32_t work = chunk_work(p, c, true, true);
   if (!(work & (1u << kWorkSC)) || !c.valid) return;
   if (work & (1u << kWorkFS1))
@@ -539,6 +539,52 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
   finish_tile(p, tid, tx, ty, old_work, sc, fs1, nonfluid, work_bits, 0u);
 }
 
+// Block 0 of the update reduces the per-wavefront boxes pass 1 wrote this
+// step and sends the result to the host (mapped memory): first the box, then
+// -- after a system-scope fence, so that a host that reads the step and then
+// the box gets a box at least as new -- the step.  Every thread of block 0
+// calls it.
+__device__ __forceinline__ void publish_box(const Params& p, uint32_t* red) {
+  const uint32_t parts = (p.work_x * p.tile_rows + 63u) / 64u;
+  uint32_t ylo = ~0u, yhi = 0u, xlo = ~0u, xhi = 0u;
+  for (uint32_t i = threadIdx.x; i < parts; i += blockDim.x) {
+    const uint32_t* e = p.box + 4u * i;
+    ylo = min(ylo, e[0]);
+    yhi = max(yhi, e[1]);
+    xlo = min(xlo, e[2]);
+    xhi = max(xhi, e[3]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ylo = min(ylo, uint32_t(__shfl_xor(int(ylo), o, 64)));
+    yhi = max(yhi, uint32_t(__shfl_xor(int(yhi), o, 64)));
+    xlo = min(xlo, uint32_t(__shfl_xor(int(xlo), o, 64)));
+    xhi = max(xhi, uint32_t(__shfl_xor(int(xhi), o, 64)));
+  }
+  const uint32_t wave = threadIdx.x / 64u, waves = blockDim.x / 64u;
+  if ((threadIdx.x & 63u) == 0) {
+    red[4 * wave] = ylo;
+    red[4 * wave + 1] = yhi;
+    red[4 * wave + 2] = xlo;
+    red[4 * wave + 3] = xhi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < waves; w++) {
+      ylo = min(ylo, red[4 * w]);
+      yhi = max(yhi, red[4 * w + 1]);
+      xlo = min(xlo, red[4 * w + 2]);
+      xhi = max(xhi, red[4 * w + 3]);
+    }
+    const unsigned long long packed =
+        (unsigned long long)(ylo & 0xffffu) | (unsigned long long)(yhi & 0xffffu) << 16 |
+        (unsigned long long)(xlo & 0xffffu) << 32 | (unsigned long long)(xhi & 0xffffu) << 48;
+    p.box_host[0] = packed;
+    __threadfence_system();
+    p.box_host[1] = (unsigned long long)p.step;
+  }
+}
+
 // The update kernel walks the tile list pass 1 built for this step.  A tile
 // left off the list was quiet in its last update and its neighbour tiles held
 // only gas and walls after the last step, so stream-collide and pass 1 left
@@ -554,6 +600,10 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   __shared__ uint32_t work_bits;
   __shared__ uint32_t busy_waves[2];  // one byte per wavefront
   static_assert(kUpdateThreads / 64 <= 8, "one byte per wavefront");
+  if (blockIdx.x == 0) {
+    __shared__ uint32_t red[4 * (kUpdateThreads / 64)];
+    publish_box(p, red);
+  }
   uint32_t i = blockIdx.x;
   uint32_t tile = first_entry(p.tile_list, i, p.work_x * p.tile_rows);
   const uint32_t count = *p.tile_count;
@@ -604,9 +654,11 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
   // classification waits for memory once.
   for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < tiles; t0 += nt) {
     const uint32_t t = t0 + lane;
-    bool need = false;
+    bool need = false, has_sc = false;
+    uint32_t ty = 0, tx = 0;
     if (t < tiles) {
-      const uint32_t ty = t / p.work_x, tx = t - ty * p.work_x;
+      ty = t / p.work_x;
+      tx = t - ty * p.work_x;
       const uint32_t xm = tx > 0 ? tx - 1 : p.work_x - 1, xp = tx + 1 < p.work_x ? tx + 1 : 0;
       const uint32_t ym = ty > 0 ? ty - 1 : p.tile_rows - 1;
       const uint32_t yp = ty + 1 < p.tile_rows ? ty + 1 : 0;
@@ -621,6 +673,7 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
       }
       const bool spill = (thin_x && (tx == 0 || tx + 2 == p.work_x)) ||
                          (thin_y && (ty == 0 || ty + 2 == p.tile_rows));
+      has_sc = ((w[4] | pub) & kWorkSCMask) != 0;
       bool skip = false, body = false;
       if (!pub && !spill) {
         uint32_t any = 0, all = ~0u;
@@ -638,6 +691,26 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
       }
       p.tile_body[t] = body;
       need = !skip && !body;
+    }
+    // Bounding box of this wavefront's tiles with stream-collide work (empty:
+    // lo > hi), one entry of Params::box per 64 tiles; the update reduces them.
+    {
+      uint32_t ylo = has_sc ? ty : ~0u, yhi = has_sc ? ty : 0u;
+      uint32_t xlo = has_sc ? tx : ~0u, xhi = has_sc ? tx : 0u;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        ylo = min(ylo, uint32_t(__shfl_xor(int(ylo), o, 64)));
+        yhi = max(yhi, uint32_t(__shfl_xor(int(yhi), o, 64)));
+        xlo = min(xlo, uint32_t(__shfl_xor(int(xlo), o, 64)));
+        xhi = max(xhi, uint32_t(__shfl_xor(int(xhi), o, 64)));
+      }
+      if (lane == 0) {
+        uint32_t* e = p.box + 4u * (t0 / 64u);
+        e[0] = ylo;
+        e[1] = yhi;
+        e[2] = xlo;
+        e[3] = xhi;
+      }
     }
     // One atomic per wavefront appends its listed tiles.
     const uint64_t m = __ballot(need);
