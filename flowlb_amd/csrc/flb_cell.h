@@ -328,7 +328,9 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
       p.trace[2 * N + n] = my;
     }
   }
-  if (fluid_dev && (DENSE || interior_fluid(fl, nf))) {
+  // A fluid cell whose eight neighbours are fluid or interface (general path).
+  const bool inner = (FS && !DENSE) ? interior_fluid(fl, nf) : true;
+  if (fluid_dev && (DENSE || inner)) {
     // Deviation m - rho that link-by-link accounting would give: the pulled
     // shares, plus the rounding of rho itself, plus (16-bit storage) the mass
     // the previous step's stores rounded away.
@@ -381,11 +383,15 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   if (MOM || FS) {
     // Free surface: every cell's density is read by pass 4, but the velocity
     // of a fluid cell only at readback -- pass 1 reads the velocity of
-    // inlet/outlet interface cells, pass 3 averages interface neighbours
-    // (a fluid cell never neighbours a gas or gas->interface cell).
-    // The density of a body-of-fluid cell is read only at readback: the cell
-    // sets its mass itself, and no gas cell borders it.
-    const bool want_u = MOM || (!DENSE && (fl & kFGI) != kFluid);
+    // inlet/outlet interface cells, pass 3 averages the fluid and interface
+    // neighbours of a gas->interface cell, which are interface cells as long
+    // as no fluid cell borders gas.  The initial flags can hold such a fluid
+    // cell where water touches an open periodic edge (the seeding does not
+    // wrap), so every fluid cell that is not interior (general path only)
+    // stores its velocity too.  The density of a body-of-fluid cell is read
+    // only at readback: the cell sets its mass itself, and no gas cell
+    // borders it.
+    const bool want_u = MOM || (!DENSE && ((fl & kFGI) != kFluid || !inner));
     if (!io) {
       if (want_u) {
         u_f[n] = ux;

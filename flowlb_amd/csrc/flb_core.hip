@@ -676,14 +676,17 @@ extern "C" {
 int flb_step(flb_handle* h, uint64_t nsteps) {
   if (!h) return FLB_ERR_ARGUMENT;
   const bool throttle = h->box_host != nullptr;
-  constexpr int K = flb_handle::kInFlight;
+  constexpr uint64_t K = flb_handle::kInFlight;
   for (uint64_t k = 0; k < nsteps; ++k) {
-    // Free surface: wait until the step K back has finished, so that the
-    // published stream-collide box is at most about K steps old.
-    if (throttle && k >= uint64_t(K)) FLB_HIP(h, hipEventSynchronize(h->in_flight[k % K]));
+    // Free surface: wait until step t - K has finished (its event slot; a slot
+    // never recorded, or recorded by an older step, returns at once), so that
+    // the published stream-collide box is at most about K steps old, across
+    // calls as well.
+    const uint64_t slot = h->t % K;
+    if (throttle) FLB_HIP(h, hipEventSynchronize(h->in_flight[slot]));
     const int rc = enqueue_step(h, k + 1 == nsteps, nullptr);
     if (rc) return rc;
-    if (throttle) FLB_HIP(h, hipEventRecord(h->in_flight[k % K], h->stream));
+    if (throttle) FLB_HIP(h, hipEventRecord(h->in_flight[slot], h->stream));
   }
   return FLB_OK;
 }

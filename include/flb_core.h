@@ -18,8 +18,11 @@
  *     5:(+1,+1) 6:(-1,-1) 7:(+1,-1) 8:(-1,+1) (src/cuda/cudaInitData.cu:76-78).
  *
  * Threading: one handle per host thread; calls on one handle are serialised by
- * the caller.  flb_step only enqueues work on the handle's HIP stream; the
- * readback calls (flb_read*, flb_mass) and flb_sync wait for it.
+ * the caller.  flb_step only enqueues work on the handle's HIP stream (a
+ * free-surface handle keeps at most 8 steps queued: the host sizes each
+ * stream-collide launch from a bounding box the device published a few steps
+ * earlier, so flb_step may wait for the step 8 back); the readback calls
+ * (flb_read*, flb_mass) and flb_sync wait for it.
  */
 #ifndef FLB_CORE_H
 #define FLB_CORE_H

@@ -163,3 +163,60 @@ def test_work_map_skips_only_idle_tiles(mass_mode):
             np.testing.assert_array_equal(ga[k], gb[k])
         np.testing.assert_array_equal(a.populations(), b.populations())
         assert (ga["flags"] != FAST.build()[0]).sum() > 100
+
+
+def _periodic_blob(nx, ny, walls_y):
+    """Gas box with a water block on the bottom edge; periodic in x, and in y
+    unless walls_y puts wall rows on the bottom and top edges."""
+    flags = np.full((ny, nx), core.GAS, np.uint8)
+    y0 = 1 if walls_y else 0
+    flags[y0:y0 + 24, 3:53] = core.FLUID
+    if walls_y:
+        flags[0, :] = core.WALL
+        flags[-1, :] = core.WALL
+    return core.seed_free_surface(flags.ravel(), nx, ny)
+
+
+@pytest.mark.parametrize("walls_y", [False, True])
+def test_launch_window_across_periodic_seams(walls_y):
+    """Stream-collide is launched over the box of chunks with work that the
+    update published, widened by one cell per step of lag.  A block of water
+    on the bottom edge of a periodic box, pulled down and to the left, crosses
+    the x seam and falls through the y seam (with wall rows closing that one,
+    it spreads along the bed under a window clamped in y); the state must equal that of a run whose window
+    is the whole grid every step (flb_set_fs_fields forgets the box), bit for
+    bit, whether the host enqueues one step per call or many."""
+    nx, ny = 256, 96
+    flags = _periodic_blob(nx, ny, walls_y)
+    # The corrected gas gate: with the gate as written the block barely moves.
+    opts = core.default_options(tau=0.6, fx=-2e-4, fy=-3e-4, use_force=1,
+                                problem=core.FREE_SURFACE, storage="fp32", mass_mode=1,
+                                gas_check_corrected=1)
+    rho0 = np.ones(nx * ny, np.float32)
+    u0 = np.zeros(2 * nx * ny, np.float32)
+    fields = core.init_free_surface_fields(flags)
+    runs = []
+    for mode in ("one_call", "per_step", "full_grid"):
+        s = core.Solver(flags, nx, ny, opts, rho0=rho0, u0=u0)
+        s.set_fs_fields(*fields)
+        if mode == "one_call":
+            s.step(400)
+        else:
+            for _ in range(400):
+                s.step(1)
+                if mode == "full_grid":
+                    g = s.read()
+                    s.set_fs_fields(g["mass"], g["phi"], g["excess"])
+        runs.append((s.read(), s.populations()))
+        s.close()
+    (ga, fa), (gb, fb), (gc, fc) = runs
+    for k in ("flags", "mass", "excess", "phi", "u", "rho"):
+        np.testing.assert_array_equal(ga[k], gc[k])
+        np.testing.assert_array_equal(gb[k], gc[k])
+    np.testing.assert_array_equal(fa, fc)
+    np.testing.assert_array_equal(fb, fc)
+    assert (ga["flags"] != flags).sum() > 500  # the water moved
+    wet = (ga["flags"].reshape(ny, nx) & (core.FLUID | core.INTERFACE)) != 0
+    assert wet[:, nx - 8:].any()  # crossed the x seam
+    if not walls_y:
+        assert wet[ny - 8:, :].any()  # fell through the bottom into the top rows
