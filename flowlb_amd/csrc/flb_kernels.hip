@@ -539,10 +539,10 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
   finish_tile(p, tid, tx, ty, old_work, sc, fs1, nonfluid, work_bits, 0u);
 }
 
-// Block 0 of the update reduces the per-wavefront boxes pass 1 wrote this
+// One block of the update reduces the per-wavefront boxes pass 1 wrote this
 // step and sends the result to the host (mapped memory): first the box, then
 // -- after a system-scope fence, so that a host that reads the step and then
-// the box gets a box at least as new -- the step.  Every thread of block 0
+// the box gets a box at least as new -- the step.  Every thread of the block
 // calls it.
 __device__ __forceinline__ void publish_box(const Params& p, uint32_t* red) {
   const uint32_t parts = (p.work_x * p.tile_rows + 63u) / 64u;
@@ -600,7 +600,9 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   __shared__ uint32_t work_bits;
   __shared__ uint32_t busy_waves[2];  // one byte per wavefront
   static_assert(kUpdateThreads / 64 <= 8, "one byte per wavefront");
-  if (blockIdx.x == 0) {
+  // The last block publishes the box: with fewer listed tiles than blocks it
+  // has no tile, so the publication does not lengthen a tile's chain.
+  if (blockIdx.x + 1 == gridDim.x) {
     __shared__ uint32_t red[4 * (kUpdateThreads / 64)];
     publish_box(p, red);
   }
