@@ -22,13 +22,15 @@
 
 namespace flb {
 
-// One block per tile.  (A grid of resident blocks looping over the tiles was
-// measured 30 % slower on the 8192 x 2048 dam break: the hardware overlaps the
-// short all-gas wavefronts of independent blocks better than a loop does.)
-// Free surface: a wavefront whose row chunk has no stream-collide work leaves
-// after reading its tile word.  DENSE (conservative mode): the kernel runs only
-// the chunks without pass-1 work -- gas, walls and interior fluid -- and
-// k_stream_collide_surface the others.
+// One block per row segment of blockDim.x cells.  (A grid of resident blocks
+// looping over the tiles was measured 30 % slower on the 8192 x 2048 dam
+// break: the hardware overlaps the short all-gas wavefronts of independent
+// blocks better than a loop does.)  Free surface: the launch covers the window
+// of Params (locate_window), and a wavefront whose row chunk has no
+// stream-collide work leaves after reading its tile word.  DENSE (conservative
+// mode, FP64 storage; the other formats use k_stream_collide_all): the kernel
+// runs only the chunks without pass-1 work -- gas, walls and interior fluid --
+// and k_stream_collide_surface the others.
 template <class S, int ODD, bool FS, bool MOM, bool TRACE = false, bool DENSE = false>
 __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
   const Cell c = FS ? locate_window(p) : locate(p);
