@@ -30,7 +30,8 @@ class Options(C.Structure):
                 ("use_force", C.c_int32), ("collision", C.c_int32), ("smagorinsky", C.c_int32),
                 ("problem", C.c_int32), ("storage", C.c_int32),
                 ("gas_check_corrected", C.c_int32), ("mass_mode", C.c_int32),
-                ("block_x", C.c_int32), ("device", C.c_int32), ("row_pitch", C.c_int32)]
+                ("block_x", C.c_int32), ("device", C.c_int32), ("row_pitch", C.c_int32),
+                ("whole_grid", C.c_int32)]
 
 
 class Rect(C.Structure):

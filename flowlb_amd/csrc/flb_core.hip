@@ -271,6 +271,7 @@ int validate(const flb_options* o, uint32_t nx, uint32_t ny, std::string& why) {
   if (o->block_x != 0 && (o->block_x < 64 || o->block_x > 1024 || o->block_x % 64))
     return why = "block_x must be a multiple of 64 in [64, 1024]", 1;
   if (nx < 2 || ny < 2) return why = "grid must be at least 2 x 2", 1;
+  if (o->whole_grid != 0 && o->whole_grid != 1) return why = "whole_grid must be 0 or 1", 1;
   if (o->row_pitch != 0 && (o->row_pitch < 0 || uint32_t(o->row_pitch) < nx))
     return why = "row_pitch must be 0 or at least nx", 1;
   if ((uint64_t(nx) + 128) * ny > 0xffffffffull / 2 ||
@@ -552,7 +553,8 @@ int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const 
 // seam; an empty box means no work at all (nblocks 0).  Without a published
 // box the window is the whole grid.
 void stream_collide_window(const flb_handle* h, flb::Params& p, int bx) {
-  if (!h->box_host || h->work_x >= 0xffffu || h->tile_rows >= 0xffffu) return;
+  if (h->opt.whole_grid || !h->box_host || h->work_x >= 0xffffu || h->tile_rows >= 0xffffu)
+    return;
   const volatile unsigned long long* bh = h->box_host;
   const unsigned long long s = bh[1];  // the step first, then the box
   if (s == ~0ull || s > h->t) return;

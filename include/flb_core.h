@@ -96,6 +96,10 @@ typedef struct flb_options {
   int32_t device;              /* HIP device ordinal */
   int32_t row_pitch;           /* cells per row of the device arrays (>= nx); 0 = nx
                                   rounded up to 64 (see flb_get_layout) */
+  int32_t whole_grid;          /* free surface: 1 launches stream-collide over the whole
+                                  grid every step instead of over the window that can hold
+                                  work (same results; a check and a fallback); 0 = window.
+                                  No reference counterpart. */
 } flb_options;
 
 typedef struct flb_handle flb_handle;

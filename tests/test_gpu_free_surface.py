@@ -177,36 +177,36 @@ def _periodic_blob(nx, ny, walls_y):
     return core.seed_free_surface(flags.ravel(), nx, ny)
 
 
+@pytest.mark.parametrize("storage", ["fp32", "fp16s", "fp64"])
 @pytest.mark.parametrize("walls_y", [False, True])
-def test_launch_window_across_periodic_seams(walls_y):
+def test_launch_window_across_periodic_seams(walls_y, storage):
     """Stream-collide is launched over the box of chunks with work that the
     update published, widened by one cell per step of lag.  A block of water
     on the bottom edge of a periodic box, pulled down and to the left, crosses
     the x seam and falls through the y seam (with wall rows closing that one,
-    it spreads along the bed under a window clamped in y); the state must equal that of a run whose window
-    is the whole grid every step (flb_set_fs_fields forgets the box), bit for
-    bit, whether the host enqueues one step per call or many."""
+    it spreads along the bed under a window clamped in y); the state must
+    equal that of a run launched over the whole grid every step
+    (flb_options.whole_grid), bit for bit, whether the host enqueues one step
+    per call or many -- for the one-launch kernel (32- and 16-bit storage) and
+    the FP64 bodies launch."""
     nx, ny = 256, 96
     flags = _periodic_blob(nx, ny, walls_y)
-    # The corrected gas gate: with the gate as written the block barely moves.
-    opts = core.default_options(tau=0.6, fx=-2e-4, fy=-3e-4, use_force=1,
-                                problem=core.FREE_SURFACE, storage="fp32", mass_mode=1,
-                                gas_check_corrected=1)
     rho0 = np.ones(nx * ny, np.float32)
     u0 = np.zeros(2 * nx * ny, np.float32)
     fields = core.init_free_surface_fields(flags)
     runs = []
-    for mode in ("one_call", "per_step", "full_grid"):
+    for mode in ("one_call", "per_step", "whole_grid"):
+        # The corrected gas gate: with the gate as written the block barely moves.
+        opts = core.default_options(tau=0.6, fx=-2e-4, fy=-3e-4, use_force=1,
+                                    problem=core.FREE_SURFACE, storage=storage, mass_mode=1,
+                                    gas_check_corrected=1, whole_grid=int(mode == "whole_grid"))
         s = core.Solver(flags, nx, ny, opts, rho0=rho0, u0=u0)
         s.set_fs_fields(*fields)
-        if mode == "one_call":
-            s.step(400)
-        else:
+        if mode == "per_step":
             for _ in range(400):
                 s.step(1)
-                if mode == "full_grid":
-                    g = s.read()
-                    s.set_fs_fields(g["mass"], g["phi"], g["excess"])
+        else:
+            s.step(400)
         runs.append((s.read(), s.populations()))
         s.close()
     (ga, fa), (gb, fb), (gc, fc) = runs
