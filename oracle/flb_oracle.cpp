@@ -837,6 +837,34 @@ void orc_set_f(void* s, const double* f) {
 double orc_total_mass(void* s) { return static_cast<Sim<double>*>(s)->total_mass(); }
 void orc_destroy(void* s) { delete static_cast<Sim<double>*>(s); }
 
+void orc_set_cells(void* s, const uint8_t* flags, const double* rho, const double* u) {
+  auto* q = static_cast<Sim<double>*>(s);
+  if (flags) std::memcpy(q->flags.data(), flags, q->N);
+  if (rho) std::memcpy(q->rho.data(), rho, q->N * sizeof(double));
+  if (u) std::memcpy(q->u.data(), u, 2 * q->N * sizeof(double));
+}
+void orc_set_time(void* s, uint64_t t) { static_cast<Sim<double>*>(s)->t = t; }
+
+int orc_run_pass(void* s, int pass) {
+  auto* q = static_cast<Sim<double>*>(s);
+  if (!q->p.free_surface && pass != 2) return 1;
+  switch (pass) {
+    case 1: q->fs1(); return 0;
+    case 2: q->stream_collide(); return 0;
+    case 3: q->fs2(); return 0;
+    case 4: q->fs3(); return 0;
+    case 5: q->fs4(); return 0;
+    default: return 1;
+  }
+}
+
+void orc_get_reconstructed(void* s, double* rec, uint16_t* mask) {
+  auto* q = static_cast<Sim<double>*>(s);
+  if (!q->p.free_surface) return;
+  if (rec) std::memcpy(rec, q->rec.data(), 9 * q->N * sizeof(double));
+  if (mask) std::memcpy(mask, q->rec_mask.data(), q->N * sizeof(uint16_t));
+}
+
 double orc_discharge(void* s, uint32_t column) {
   auto* q = static_cast<Sim<double>*>(s);
   double acc = 0.0;

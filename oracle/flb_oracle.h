@@ -76,6 +76,19 @@ void orc_set_f(void* s, const double* f);
 double orc_total_mass(void* s);
 void orc_destroy(void* s);
 
+/* Single passes on a hand-built state, for the tests that pin each kernel.
+   orc_set_cells overwrites flags / rho / u (each pointer may be NULL);
+   orc_set_time sets the lattice time that selects the slot parity.
+   orc_run_pass runs one kernel without advancing time: 1 d_FreeSurface2D_1,
+   2 d_StreamCollide2D, 3 d_FreeSurface2D_2, 4 d_FreeSurface2D_3,
+   5 d_FreeSurface2D_4; it returns nonzero for an unknown pass.
+   orc_get_reconstructed reads the populations the last pass 1 reconstructed
+   (rec[k*N + n]) and, per cell, the bit mask of the directions k it replaced. */
+void orc_set_cells(void* s, const uint8_t* flags, const double* rho, const double* u);
+void orc_set_time(void* s, uint64_t t);
+int orc_run_pass(void* s, int pass);
+void orc_get_reconstructed(void* s, double* rec, uint16_t* mask);
+
 /* Discharge through a column: sum over non-wall cells of rho*ux using the
    moments of the last step. */
 double orc_discharge(void* s, uint32_t column);

@@ -75,6 +75,11 @@ def lib():
         L.orc_time.argtypes = [P]
         L.orc_time.restype = C.c_uint64
         L.orc_set_f.argtypes = [P, dp]
+        L.orc_set_cells.argtypes = [P, u8p, dp, dp]
+        L.orc_set_time.argtypes = [P, C.c_uint64]
+        L.orc_run_pass.argtypes = [P, C.c_int]
+        L.orc_run_pass.restype = C.c_int
+        L.orc_get_reconstructed.argtypes = [P, dp, C.POINTER(C.c_uint16)]
         L.orc_discharge.argtypes = [P, C.c_uint32]
         L.orc_discharge.restype = C.c_double
         if hasattr(L, "orc_adjoint"):
@@ -177,6 +182,28 @@ class Oracle:
 
     def total_mass(self):
         return getattr(lib(), self.pre + "_total_mass")(self.h)
+
+    # Single passes on a hand-built state (FP64 oracle only).
+    PASSES = {"fs1": 1, "stream_collide": 2, "fs2": 3, "fs3": 4, "fs4": 5}
+
+    def set_cells(self, flags=None, rho=None, u=None):
+        flags = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        rho = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+        u = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+        lib().orc_set_cells(self.h, None if flags is None else _u8(flags), _dp(rho), _dp(u))
+
+    def set_time(self, t):
+        lib().orc_set_time(self.h, t)
+
+    def run_pass(self, name):
+        if self.pre != "orc" or lib().orc_run_pass(self.h, self.PASSES[name]):
+            raise ValueError(f"pass {name} not available on this oracle")
+
+    def reconstructed(self):
+        rec = np.zeros(9 * self.N)
+        mask = np.zeros(self.N, dtype=np.uint16)
+        lib().orc_get_reconstructed(self.h, _dp(rec), mask.ctypes.data_as(C.POINTER(C.c_uint16)))
+        return rec.reshape(9, self.N), mask
 
     def time(self):
         return lib().orc_time(self.h)
