@@ -56,15 +56,16 @@ def algorithmic_bytes(active, inactive, s, free_surface):
     return sc, step
 
 
-def cpu_baseline(case, seconds=15.0):
-    """FP64 serial oracle on the full grid of the workload, a few steps."""
+def _oracle_rate(case, fp32, threads, seconds):
+    """MLUPS of the oracle on the full grid of the workload: steps after one
+    warm-up step until `seconds` have passed (at most 1000)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
     flags, rho0, u0 = case.build()
     o = pyoracle.Oracle(flags, case.nx, case.ny, case.tau, rho0=rho0.astype(np.float64),
                         u0=u0.astype(np.float64), use_force=case.force != (0.0, 0.0),
                         fx=case.force[0], fy=case.force[1], free_surface=case.free_surface,
-                        mass_mode=1, threads=1)
+                        mass_mode=1, threads=threads, fp32=fp32)
     o.step(1)
     n, t0 = 0, time.perf_counter()
     while True:
@@ -74,20 +75,57 @@ def cpu_baseline(case, seconds=15.0):
         if dt > seconds or n >= 1000:
             break
     o.close()
-    return dict(value=round(case.N * n / dt / 1e6, 3), unit="MLUPS", cores=1, kind="port",
-                sample=f"FP64 serial oracle, {n} steps of the full {case.nx}x{case.ny} grid "
-                       f"after 1 warm-up step ({dt:.1f} s)")
+    return case.N * n / dt / 1e6, n, dt
 
 
-def measure(case, storage, steps, warmup, profile_steps, **opt):
+def cpu_baseline(case, seconds=10.0):
+    """The oracle timed on this host: FP32 with OpenMP on all cores of the
+    host's share (the headline CPU figure), FP32 on one core (for the parallel
+    efficiency) and FP64 on one core, each on a bounded sample of the
+    workload's full grid; plus the 512 x 128 single-phase culvert run in full
+    (10^4 steps, FP64, one core)."""
+    from flowlb_amd import cases
+    cores = min(16, os.cpu_count() or 1)
+    par, npar, dpar = _oracle_rate(case, True, cores, seconds)
+    one, n1, d1 = _oracle_rate(case, True, 1, seconds)
+    f64, n64, d64 = _oracle_rate(case, False, 1, seconds)
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    c1 = cases.SINGLE_PHASE
+    flags, rho0, u0 = c1.build()
+    o = pyoracle.Oracle(flags, c1.nx, c1.ny, c1.tau, rho0=rho0.astype(np.float64),
+                        u0=u0.astype(np.float64), threads=1)
+    t0 = time.perf_counter()
+    o.step(c1.steps)
+    c1_s = time.perf_counter() - t0
+    o.close()
+    return dict(
+        value=round(par, 3), unit="MLUPS", cores=cores, kind="port",
+        sample=f"FP32 oracle with OpenMP on {cores} threads, {npar} steps of the full "
+               f"{case.nx}x{case.ny} grid after 1 warm-up step ({dpar:.1f} s)",
+        fp32_one_core={"value": round(one, 3), "steps": n1, "seconds": round(d1, 2)},
+        parallel_efficiency=round(par / (one * cores), 3),
+        fp64_one_core={"value": round(f64, 3), "steps": n64, "seconds": round(d64, 2)},
+        single_phase_512x128_full_run={"steps": c1.steps, "seconds": round(c1_s, 2),
+                                       "MLUPS": round(c1.N * c1.steps / c1_s / 1e6, 3),
+                                       "precision": "fp64", "cores": 1})
+
+
+def measure(case, storage, steps, warmup, profile_steps, t0=0, **opt):
+    """Advances the workload to lattice time t0 and then `warmup` steps, times
+    `steps` steps (HIP events on the solver stream), then `profile_steps` with
+    every kernel bracketed by events (the per-kernel means)."""
     s = case.solver(storage, **opt)
     try:
+        if t0:
+            s.step(t0)
         s.step(warmup)
         s.sync()
         a0, i0 = active_counts(s)
-        t0 = time.perf_counter()
+        t_start = s.time
+        w0 = time.perf_counter()
         ms = s.step_timed(steps)
-        wall = time.perf_counter() - t0
+        wall = time.perf_counter() - w0
         a1, i1 = active_counts(s)
         active, inactive = (a0 + a1) / 2, (i0 + i1) / 2
         tot_k, per_k = s.step_timed(profile_steps, per_kernel=True)
@@ -99,7 +137,7 @@ def measure(case, storage, steps, warmup, profile_steps, **opt):
     sc_ms = per_k[0] / profile_steps
     return dict(ms=ms, wall=wall, active=active, inactive=inactive, sc_bytes=sc_b,
                 step_bytes=step_b, sc_ms=sc_ms, per_kernel_ms=[v / profile_steps for v in per_k],
-                finite=bool(mass_ok))
+                finite=bool(mass_ok), t_start=int(t_start))
 
 
 def hbm_reference():
@@ -283,6 +321,91 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def _fs_discharge(g, nx, ny, column):
+    """sum of phi rho u_x over the wet (fluid or interface) cells of a column."""
+    fl = g["flags"].reshape(ny, nx)[:, column]
+    wet = (fl & 0x05) != 0
+    phi = g["phi"].reshape(ny, nx)[:, column].astype(np.float64)
+    rho = g["rho"].reshape(ny, nx)[:, column].astype(np.float64)
+    ux = g["u"][:nx * ny].reshape(ny, nx)[:, column].astype(np.float64)
+    return float(np.sum(np.where(wet, phi * rho * ux, 0.0)))
+
+
+def _front(flags, nx, ny):
+    wet = (flags.reshape(ny, nx) & 0x05) != 0
+    return int(np.nonzero(wet.any(axis=0))[0].max())
+
+
+def _surface_mean(phi, nx, ny, k=8):
+    """Per column the highest cell with phi >= 0.5 (+ its linear crossing),
+    averaged over groups of k columns."""
+    p = phi.reshape(ny, nx)
+    full = p >= 0.5
+    top = np.where(full.any(axis=0), ny - 1 - np.argmax(full[::-1], axis=0), 0)
+    h = top.astype(np.float64)
+    cols = np.arange(nx)
+    above = np.minimum(top + 1, ny - 1)
+    a, b = p[top, cols], p[above, cols]
+    ok = full.any(axis=0) & (top + 1 < ny) & (a > b)
+    h[ok] += (a[ok] - 0.5) / (a[ok] - b[ok])
+    return h.reshape(-1, k).mean(axis=1)
+
+
+def bench_accuracy(args):
+    """Storage formats against the FP64 oracle on the full 8192 x 2048 culvert:
+    the collapsing column (corrected gas gate) for 3000 steps; time-averaged
+    discharge through column 3620 (the front crosses it before step 1000;
+    mean over steps 1000-3000, every 100), front position and the surface
+    profile (8-column means).  The FP64 oracle runs on the host's cores."""
+    from flowlb_amd import cases
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    case = cases.FS_8192
+    nx, ny = case.nx, case.ny
+    col, t_from, every, steps = 3620, 1000, 100, 3000
+    samples = list(range(t_from, steps + 1, every))
+    res = {}
+    for st in ("fp32", "fp16s", "bf16s"):
+        qs = []
+        with case.solver(st, mass_mode=1, gas_check_corrected=1) as s:
+            t = 0
+            for ts in samples:
+                s.step(ts - t)
+                t = ts
+                g = s.read()
+                qs.append(_fs_discharge(g, nx, ny, col))
+        res[st] = dict(q=float(np.mean(qs)), front=_front(g["flags"], nx, ny),
+                       surface=_surface_mean(g["phi"], nx, ny), u=g["u"][:nx * ny])
+        print(f"gpu {st}: discharge {res[st]['q']:.6e}, front {res[st]['front']}", flush=True)
+    threads = min(16, os.cpu_count() or 1)
+    flags, _, _ = case.build()
+    o = pyoracle.Oracle(flags, nx, ny, case.tau, use_force=True, fx=case.force[0], fy=case.force[1],
+                        free_surface=True, mass_mode=1, gas_check_corrected=True, threads=threads)
+    qs, t, t0 = [], 0, time.perf_counter()
+    for ts in samples:
+        while t < ts:
+            o.step(every)
+            t += every
+            print(f"oracle t={t} ({time.perf_counter() - t0:.0f} s)", flush=True)
+        qs.append(o.fs_discharge(col))
+    r = o.get()
+    o.close()
+    q64, front64 = float(np.mean(qs)), _front(r["flags"], nx, ny)
+    surf64 = _surface_mean(r["phi"], nx, ny)
+    out = {"metric": "storage accuracy vs FP64 oracle", "config": {
+        "workload": case.name, "nx": nx, "ny": ny, "steps": steps, "gas_check_corrected": 1,
+        "discharge_column": col, "discharge_steps": [t_from, steps, every]},
+        "oracle": {"discharge": q64, "front": front64, "threads": threads,
+                   "seconds": round(time.perf_counter() - t0, 1)}}
+    for st, v in res.items():
+        out[st] = {"discharge": v["q"], "discharge_rel_err": abs(v["q"] - q64) / abs(q64),
+                   "front": v["front"], "front_err_cells": v["front"] - front64,
+                   "surface_mean_abs_err_cells": float(np.abs(v["surface"] - surf64).mean()),
+                   "u_x_relL2": float(np.linalg.norm(v["u"] - r["u"][:nx * ny])
+                                      / np.linalg.norm(r["u"][:nx * ny]))}
+    print(json.dumps(out), flush=True)
+
+
 def copy_probe(args):
     """A plain device copy of known size (torch, 1 GiB read + 1 GiB written per
     launch): the reference point that scales the FETCH_SIZE / WRITE_SIZE
@@ -310,7 +433,12 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-hbm-reference", action="store_true",
                     help="skip the copy / in-place rates measured in a child process")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--t0", type=int, default=1000,
+                    help="lattice time the workload is advanced to before the warm-up, so "
+                         "that the timed steps start at t0 + warmup whatever --warmup is")
+    ap.add_argument("--no-moving", action="store_true",
+                    help="skip the free-surface line under a moving interface")
     ap.add_argument("--fd-cells", type=int, default=16)
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--single-steps", type=int, default=20)
@@ -329,6 +457,8 @@ def main():
         return bench_adjoint(args)
     if args.workload.startswith("batch"):
         return bench_batch(args)
+    if args.workload == "accuracy_8192x2048":
+        return bench_accuracy(args)
     # The child runs before this process loads the solver library or touches
     # the GPU -- and not at all under a profiler, whose preloaded library has
     # started the GPU runtime in this process already.
@@ -337,7 +467,12 @@ def main():
     hbm_ref = {} if args.no_hbm_reference or profiled else hbm_reference_child()
     from flowlb_amd import cases, core
     case = cases.CASES[args.workload]
-    r = measure(case, args.storage, args.steps, args.warmup, args.profile_steps)
+    probes = {}
+    if not args.no_hbm_reference:
+        for kind in core.PROBE_KINDS:
+            pms, pbytes = core.bandwidth_probe(case.nx, case.ny, kind, 20)
+            probes[f"{kind}_GBs"] = round(pbytes / (pms * 1e-3) / 1e9, 1)
+    r = measure(case, args.storage, args.steps, args.warmup, args.profile_steps, t0=args.t0)
     ms_step = r["ms"] / args.steps
     mlups = case.N / (ms_step * 1e-3) / 1e6
     achieved = r["sc_bytes"] / (r["sc_ms"] * 1e-3) / 1e9
@@ -369,7 +504,12 @@ def main():
                                                    "and bodies of fluid)" if case.free_surface
                                                    else ""),
                      "algorithmic_bytes_per_launch": int(r["sc_bytes"]),
-                     "avg_launch_ms": round(r["sc_ms"], 5)},
+                     "avg_launch_ms": round(r["sc_ms"], 5),
+                     "timing": f"HIP events on the solver stream around each stream-collide "
+                               f"launch of {args.profile_steps} steps right after the timed "
+                               f"region"},
+        "lattice_time": [r["t_start"], r["t_start"] + args.steps],
+        "active_fraction": round(r["active"] / case.N, 4),
         "mlups_active_cells": round(r["active"] / (ms_step * 1e-3) / 1e6, 1),
         "step_roofline": {"algorithmic_bytes_per_step": int(r["step_bytes"]),
                           "achieved_GBs": round(r["step_bytes"] / (ms_step * 1e-3) / 1e9, 1),
@@ -377,8 +517,9 @@ def main():
         "kernel_ms": {k: round(v, 5) for k, v in zip(core.KERNEL_NAMES, r["per_kernel_ms"])},
         "wall_s_timed_region": round(r["wall"], 4),
     }
-    if hbm_ref:
+    if hbm_ref or probes:
         ref = dict(hbm_ref)
+        ref.update(probes)
         if traffic:
             ref["stream_collide_measured_traffic_GBs"] = round(traffic / (r["sc_ms"] * 1e-3) / 1e9, 1)
         out["hbm_reference"] = ref
@@ -388,10 +529,21 @@ def main():
             if st == args.storage:
                 sweep[st] = out["value"]
                 continue
-            q = measure(case, st, max(50, args.steps // 2), args.warmup // 2, 10)
+            q = measure(case, st, max(50, args.steps // 2), args.warmup // 2, 10, t0=args.t0)
             sweep[st] = round(case.N / (q["ms"] / max(50, args.steps // 2) * 1e-3) / 1e6, 1)
         out["storage_mlups"] = sweep
         out["fp16s_speedup"] = round(sweep["fp16s"] / sweep["fp32"], 3)
+    if case.free_surface and not args.no_moving:
+        # The reference's gas gate keeps the column almost still; with the
+        # corrected gate it collapses: timed after 2000 steps of collapse.
+        m = measure(case, args.storage, 200, 0, 10, t0=2000, gas_check_corrected=1)
+        mms = m["ms"] / 200
+        out["moving_interface"] = {
+            "gas_check_corrected": 1, "lattice_time": [m["t_start"], m["t_start"] + 200],
+            "MLUPS": round(case.N / (mms * 1e-3) / 1e6, 1),
+            "mlups_active_cells": round(m["active"] / (mms * 1e-3) / 1e6, 1),
+            "active_fraction": round(m["active"] / case.N, 4), "ms_per_step": round(mms, 5),
+            "kernel_ms": {k: round(v, 5) for k, v in zip(core.KERNEL_NAMES, m["per_kernel_ms"])}}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -13,7 +13,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libflbcore.so"
 OBJ = PKG / "_build"
-SOURCES = ["flb_core.hip", "flb_kernels.hip", "flb_adjoint.hip", "flb_batch.hip", "flb_setup.cpp", "flb_io.cpp", "flb_deck.cpp"]
+SOURCES = ["flb_core.hip", "flb_kernels.hip", "flb_adjoint.hip", "flb_batch.hip", "flb_view.hip", "flb_probe.hip", "flb_setup.cpp", "flb_io.cpp", "flb_deck.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -75,7 +75,13 @@ ADJOINT = Case("adjoint_1024x256", 1024, 256, False, (core.INLET, core.OUTLET, W
                64, u_inlet=0.04, tau=0.6, force=(1e-6, 0.0), steps=2000,
                extra=dict(window=(336, 399, 1, 64), t_avg=(1500, 1999), ckpt=100))
 
-CASES = {c.name: c for c in (SINGLE_PHASE, FS_4096, FS_8192, FS_1024, ADJOINT)}
+# The 512 x 128 single-phase culvert scaled by 16 (fully wet apart from the
+# embankment): the single-phase stream-collide kernel at HBM scale.
+SINGLE_PHASE_8192 = Case("single_phase_8192x2048", 8192, 2048, False,
+                         (core.INLET, core.OUTLET, W, W), (3584, 4608), 512, u_inlet=0.05,
+                         rho_noise=1e-4, tau=0.56, steps=10000)
+
+CASES = {c.name: c for c in (SINGLE_PHASE, FS_4096, FS_8192, FS_1024, ADJOINT, SINGLE_PHASE_8192)}
 
 
 def scaled(case: Case, div: int, name=None) -> Case:

@@ -34,6 +34,15 @@ class Options(C.Structure):
                 ("whole_grid", C.c_int32)]
 
 
+class IsoView(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("x_min", C.c_uint32),
+                ("x_max", C.c_uint32), ("y_min", C.c_uint32), ("y_max", C.c_uint32),
+                ("iso_value", C.c_float), ("field", C.c_int32), ("clear", C.c_int32)]
+
+
+ISO_VELOCITY, ISO_PHI = 0, 1
+
+
 class Rect(C.Structure):
     _fields_ = [("x0", C.c_double), ("y0", C.c_double), ("height", C.c_double),
                 ("width", C.c_double)]
@@ -119,6 +128,7 @@ PROTOTYPES = [
     ("flb_create", C.c_int, [C.POINTER(Options), _U8, C.c_uint32, C.c_uint32, _F, _F,
                              C.POINTER(_H)]),
     ("flb_set_fs_fields", C.c_int, [_H, _F, _F, _F]),
+    ("flb_set_fs_fields_f64", C.c_int, [_H, _D, _D, _D]),
     ("flb_step", C.c_int, [_H, C.c_uint64]),
     ("flb_sync", C.c_int, [_H]),
     ("flb_read", C.c_int, [_H, _F, _F, _U8, _F]),
@@ -128,9 +138,14 @@ PROTOTYPES = [
     ("flb_read_fs_f64", C.c_int, [_H, _D, _D]),
     ("flb_read_populations_f64", C.c_int, [_H, _D]),
     ("flb_write_populations", C.c_int, [_H, _F]),
+    ("flb_write_populations_f64", C.c_int, [_H, _D]),
     ("flb_device_fields", C.c_int, [_H, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                      C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     ("flb_mass", C.c_int, [_H, _D]),
+    ("flb_export_velocity", C.c_int, [_H, C.c_void_p, C.c_size_t]),
+    ("flb_bandwidth_probe", C.c_int, [C.c_int32, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32,
+                                      _F, _D]),
+    ("flb_isosurface", C.c_int, [_H, C.POINTER(IsoView), C.c_void_p, C.c_size_t]),
     ("flb_time", C.c_uint64, [_H]),
     ("flb_step_timed", C.c_int, [_H, C.c_uint64, _F, _F]),
     ("flb_flag_histogram", C.c_int, [_H, _U64]),
@@ -176,6 +191,81 @@ PROTOTYPES = [
 ]
 
 _lib = None
+_hip = None
+
+
+def hip():
+    """The HIP runtime the core library itself links (libamdhip64), for the
+    device buffers a caller hands to the viewer entry points."""
+    global _hip
+    if _hip is None:
+        lib()
+        H = C.CDLL("libamdhip64.so.7")
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        H.hipDeviceSynchronize.argtypes = []
+        _hip = H
+    return _hip
+
+
+PROBE_KINDS = {"esoteric_inplace": 0, "copy9": 1, "inplace9": 2}
+
+
+def bandwidth_probe(nx, ny, kind, launches=20, device=0):
+    """Device time per launch (ms) and bytes moved per launch of the
+    stream-collide memory pattern without arithmetic (flb_bandwidth_probe)."""
+    ms, nbytes = C.c_float(), C.c_double()
+    check(lib().flb_bandwidth_probe(device, nx, ny, PROBE_KINDS[kind], launches, C.byref(ms),
+                                    C.byref(nbytes)), None, "flb_bandwidth_probe")
+    return ms.value, nbytes.value
+
+
+class DeviceBuffer:
+    """A plain device allocation (hipMalloc) with host copies, standing in for
+    the GL pixel buffer a viewer maps through HIP-GL interop."""
+    H2D, D2H = 1, 2
+
+    def __init__(self, nbytes, fill=None):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        if hip().hipMalloc(C.byref(p), self.nbytes) != 0:
+            raise FlbError(-1, f"hipMalloc of {self.nbytes} bytes failed")
+        self.ptr = p.value
+        if fill is not None and hip().hipMemset(C.c_void_p(self.ptr), int(fill), self.nbytes):
+            raise FlbError(-1, "hipMemset failed")
+
+    def to_host(self, dtype=np.uint8):
+        out = np.empty(self.nbytes, np.uint8)
+        if hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes,
+                           self.D2H):
+            raise FlbError(-1, "hipMemcpy to host failed")
+        return out.view(dtype)
+
+    def from_host(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        if hip().hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes, self.H2D):
+            raise FlbError(-1, "hipMemcpy to device failed")
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            hip().hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def lib():
@@ -262,6 +352,17 @@ class Solver:
         check(lib().flb_set_fs_fields(self.h, fptr(mass), fptr(phi), fptr(excess)), self.h,
               "flb_set_fs_fields")
 
+    def set_fs_fields64(self, mass, phi, excess=None):
+        """set_fs_fields from doubles (exact for FP64 storage)."""
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1)  # noqa: E731
+        mass, phi, excess = f64(mass), f64(phi), f64(excess)
+        for a in (mass, phi, excess):
+            if a is not None and a.size != self.N:
+                raise ValueError(f"expected {self.N} values, got {a.size}")
+        P = lambda a: None if a is None else a.ctypes.data_as(_D)  # noqa: E731
+        check(lib().flb_set_fs_fields_f64(self.h, P(mass), P(phi), P(excess)), self.h,
+              "flb_set_fs_fields_f64")
+
     # -- advance
     def step(self, n=1):
         check(lib().flb_step(self.h, n), self.h, "flb_step")
@@ -337,6 +438,13 @@ class Solver:
         f = _f32(f, 9 * self.N)
         check(lib().flb_write_populations(self.h, fptr(f)), self.h, "flb_write_populations")
 
+    def write_populations64(self, f):
+        f = np.ascontiguousarray(f, np.float64).reshape(-1)
+        if f.size != 9 * self.N:
+            raise ValueError(f"expected {9 * self.N} values, got {f.size}")
+        check(lib().flb_write_populations_f64(self.h, f.ctypes.data_as(_D)), self.h,
+              "flb_write_populations_f64")
+
     def mass(self):
         m = C.c_double()
         check(lib().flb_mass(self.h, C.byref(m)), self.h, "flb_mass")
@@ -346,6 +454,30 @@ class Solver:
         c = np.zeros(256, np.uint64)
         check(lib().flb_flag_histogram(self.h, c.ctypes.data_as(_U64)), self.h, "flb_flag_histogram")
         return c
+
+    # -- viewer output (device buffers owned by the caller)
+    def export_velocity(self, dst, pitch_bytes=None):
+        """Writes (ux, uy) float2 texels into the DeviceBuffer dst (rows of
+        pitch_bytes, default 8 * nx) and waits for it."""
+        pitch = 8 * self.nx if pitch_bytes is None else pitch_bytes
+        if dst.nbytes < pitch * self.ny:
+            raise ValueError("destination too small")
+        check(lib().flb_export_velocity(self.h, C.c_void_p(dst.ptr), pitch), self.h,
+              "flb_export_velocity")
+        self.sync()
+
+    def isosurface(self, tex, width, height, bounds, iso, field=ISO_VELOCITY, clear=True,
+                   pitch_bytes=None):
+        """Marching-squares isoline raster into the DeviceBuffer tex (rows of
+        pitch_bytes, default width); bounds = (x_min, x_max, y_min, y_max)."""
+        pitch = width if pitch_bytes is None else pitch_bytes
+        if tex.nbytes < pitch * height:
+            raise ValueError("texture buffer too small")
+        v = IsoView(width, height, bounds[0], bounds[1], bounds[2], bounds[3], iso, field,
+                    int(clear))
+        check(lib().flb_isosurface(self.h, C.byref(v), C.c_void_p(tex.ptr), pitch), self.h,
+              "flb_isosurface")
+        self.sync()
 
     def device_fields(self):
         ptrs = [C.c_void_p() for _ in range(4)]

@@ -529,18 +529,37 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   return FLB_OK;
 }
 
-int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const float* excess) {
+}  // extern "C"
+
+namespace {
+
+// The 16-bit storage residual that stream-collide keeps in plane 0 of rec
+// belongs to the stored populations, which new masses do not change, so it
+// is kept here (flb_write_populations clears it).
+template <class T>
+int set_fs_fields(flb_handle* h, const T* mass, const T* phi, const T* excess) {
   if (!h || !mass || !phi) return h ? fail(h, FLB_ERR_ARGUMENT, "mass and phi required") : FLB_ERR_ARGUMENT;
   if (h->opt.problem != FLB_FREE_SURFACE) return fail(h, FLB_ERR_STATE, "not a free-surface handle");
   int rc;
   if ((rc = upload_field(h, h->mass, mass, 1))) return rc;
   if ((rc = upload_field(h, h->phi, phi, 1))) return rc;
   if ((rc = reset_work(h))) return rc;  // new masses: evaluate everything next step
-  if ((rc = zero_field(h, h->rec, h->Nd))) return rc;  // stream-collide's mass residuals
   // Shares pulled by the next step; the other buffer receives the next shares.
   if ((rc = zero_field(h, h->excess[h->cur ^ 1], h->Nd))) return rc;
   if (excess) return upload_field(h, h->excess[h->cur], excess, 1);
   return zero_field(h, h->excess[h->cur], h->Nd);
+}
+
+}  // namespace
+
+extern "C" {
+
+int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const float* excess) {
+  return set_fs_fields(h, mass, phi, excess);
+}
+int flb_set_fs_fields_f64(flb_handle* h, const double* mass, const double* phi,
+                          const double* excess) {
+  return set_fs_fields(h, mass, phi, excess);
 }
 
 // Window of the free-surface stream-collide launch for step h->t.  The last
@@ -803,28 +822,48 @@ int flb_read_fs_f64(flb_handle* h, double* mass, double* excess) {
 int flb_read_populations(flb_handle* h, float* f) { return read_pops(h, f); }
 int flb_read_populations_f64(flb_handle* h, double* f) { return read_pops(h, f); }
 
-int flb_write_populations(flb_handle* h, const float* f) {
+}  // extern "C"
+
+namespace {
+
+// Shifted populations f - w in T, converted to the storage format on the
+// device.  A free-surface handle's 16-bit storage residual (plane 0 of rec)
+// belonged to the replaced populations and is cleared.
+template <class T>
+int write_pops(flb_handle* h, const T* f) {
   if (!h || !f) return FLB_ERR_ARGUMENT;
   const size_t N = h->N;
-  std::vector<float> fsh(9 * N);
+  std::vector<T> fsh(9 * N);
   for (int q = 0; q < 9; ++q) {
-    const float w = q == 0 ? 4.0f / 9.0f : (q < 5 ? 1.0f / 9.0f : 1.0f / 36.0f);
+    const T w = q == 0 ? T(4) / T(9) : (q < 5 ? T(1) / T(9) : T(1) / T(36));
     for (size_t n = 0; n < N; ++n) fsh[q * N + n] = f[q * N + n] - w;
   }
-  float* tmp = nullptr;
-  FLB_HIP(h, hipMalloc(&tmp, 9 * N * sizeof(float)));
-  int rc = upload(h, tmp, fsh.data(), 9 * N * sizeof(float));
+  T* tmp = nullptr;
+  FLB_HIP(h, hipMalloc(&tmp, 9 * N * sizeof(T)));
+  int rc = upload(h, tmp, fsh.data(), 9 * N * sizeof(T));
   if (!rc) {
     const flb::Params p = params(h, kUtilBlock);
-    hipLaunchKernelGGL(flb::get_f32_to_pop(h->opt.storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
-                       h->stream, p, tmp);
+    if (std::is_same<T, double>::value)
+      hipLaunchKernelGGL(flb::get_f64_to_pop(h->opt.storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
+                         h->stream, p, reinterpret_cast<const double*>(tmp));
+    else
+      hipLaunchKernelGGL(flb::get_f32_to_pop(h->opt.storage), dim3(p.nblocks), dim3(kUtilBlock), 0,
+                         h->stream, p, reinterpret_cast<const float*>(tmp));
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) rc = hip_fail(h, e, "f32_to_pop");
+    if (e != hipSuccess) rc = hip_fail(h, e, "write_populations");
   }
   (void)hipFree(tmp);
+  if (!rc && h->opt.problem == FLB_FREE_SURFACE) rc = zero_field(h, h->rec, h->Nd);
   return rc;
 }
+
+}  // namespace
+
+extern "C" {
+
+int flb_write_populations(flb_handle* h, const float* f) { return write_pops(h, f); }
+int flb_write_populations_f64(flb_handle* h, const double* f) { return write_pops(h, f); }
 
 int flb_device_fields(flb_handle* h, void** d_u, void** d_rho, void** d_flags, void** d_phi) {
   if (!h) return FLB_ERR_ARGUMENT;

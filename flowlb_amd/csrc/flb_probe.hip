@@ -1,0 +1,123 @@
+// flb_probe.hip -- attainable-bandwidth reference for the stream-collide
+// kernel: the same memory pattern with no arithmetic, on a buffer of its own.
+//
+// FLB_PROBE_ESOTERIC moves every cell's nine populations through the
+// esoteric-pull slots exactly as stream-collide does (nine loads from the
+// cell's load slots, nine stores to its store slots, in place, parity
+// alternating per launch), with the identity as "collision".  FLB_PROBE_COPY
+// copies nine planes to nine other planes, FLB_PROBE_INPLACE adds 1 to nine
+// planes in place.  All use FP32, one thread per cell, 256-cell row blocks in
+// stream-collide's band order, and move 72 bytes per cell (36 read, 36
+// written).  No reference counterpart (the reference reports no bandwidth).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "flb_cell.h"
+#include "flb_internal.h"
+
+namespace {
+
+constexpr int kProbeBlock = 256;
+
+struct ProbeArgs {
+  flb::Params p;  // nx, ny, pitch, tiles_x, Np used
+  float* f;
+  float* g;       // second buffer (copy)
+};
+
+template <int KIND, int ODD>
+__global__ void __launch_bounds__(kProbeBlock) k_probe(const ProbeArgs a) {
+  uint32_t tx, y;
+  flb::block_tile(blockIdx.x, a.p.tiles_x, a.p.ny, tx, y);
+  const uint32_t x = tx * kProbeBlock + threadIdx.x;
+  if (x >= a.p.nx) return;
+  const uint64_t Np = a.p.Np;
+  if (KIND == FLB_PROBE_ESOTERIC) {
+    uint32_t nb[9];
+    flb::neighbours(a.p, x, y, nb);
+    float v[9];
+    v[0] = a.f[nb[0]];
+#pragma unroll
+    for (int i = 1; i < 9; i += 2) {
+      v[i] = a.f[(ODD ? i : i + 1) * Np + nb[0]];
+      v[i + 1] = a.f[(ODD ? i + 1 : i) * Np + nb[i]];
+    }
+    a.f[nb[0]] = v[0];
+#pragma unroll
+    for (int i = 1; i < 9; i += 2) {
+      a.f[(ODD ? i + 1 : i) * Np + nb[i]] = v[i];
+      a.f[(ODD ? i : i + 1) * Np + nb[0]] = v[i + 1];
+    }
+  } else {
+    const uint64_t n = x + uint64_t(y) * a.p.pitch;
+    float v[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) v[q] = a.f[q * Np + n];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      if (KIND == FLB_PROBE_COPY) a.g[q * Np + n] = v[q];
+      else a.f[q * Np + n] = v[q] + 1.0f;
+    }
+  }
+}
+
+template <int KIND>
+void launch_probe(const ProbeArgs& a, uint32_t blocks, int odd, hipStream_t s) {
+  if (odd) hipLaunchKernelGGL((k_probe<KIND, 1>), dim3(blocks), dim3(kProbeBlock), 0, s, a);
+  else hipLaunchKernelGGL((k_probe<KIND, 0>), dim3(blocks), dim3(kProbeBlock), 0, s, a);
+}
+
+}  // namespace
+
+extern "C" int flb_bandwidth_probe(int32_t device, uint32_t nx, uint32_t ny, int32_t kind,
+                                   uint32_t launches, float* ms_per_launch,
+                                   double* bytes_per_launch) {
+  if (nx < 2 || ny < 2 || launches == 0 || !ms_per_launch ||
+      (kind != FLB_PROBE_ESOTERIC && kind != FLB_PROBE_COPY && kind != FLB_PROBE_INPLACE))
+    return FLB_ERR_ARGUMENT;
+  if (hipSetDevice(device) != hipSuccess) return FLB_ERR_DEVICE;
+  ProbeArgs a{};
+  a.p.nx = nx;
+  a.p.ny = ny;
+  a.p.pitch = (nx + 63u) / 64u * 64u;
+  a.p.tiles_x = (nx + kProbeBlock - 1) / kProbeBlock;
+  a.p.Np = (uint64_t(a.p.pitch) * ny + 63u) / 64u * 64u;
+  const size_t bytes = 9 * a.p.Np * sizeof(float);
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = FLB_OK;
+  if (hipMalloc(&a.f, bytes) != hipSuccess) return FLB_ERR_MEMORY;
+  if (kind == FLB_PROBE_COPY && hipMalloc(&a.g, bytes) != hipSuccess) rc = FLB_ERR_MEMORY;
+  if (!rc && (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+              hipEventCreate(&e1) != hipSuccess))
+    rc = FLB_ERR_DEVICE;
+  if (!rc && hipMemsetAsync(a.f, 0, bytes, s) != hipSuccess) rc = FLB_ERR_DEVICE;
+  if (!rc) {
+    const uint32_t blocks = a.p.tiles_x * ny;
+    auto run = [&](uint32_t k) {
+      for (uint32_t i = 0; i < k; ++i) {
+        const int odd = int(i & 1u);
+        if (kind == FLB_PROBE_ESOTERIC) launch_probe<FLB_PROBE_ESOTERIC>(a, blocks, odd, s);
+        else if (kind == FLB_PROBE_COPY) launch_probe<FLB_PROBE_COPY>(a, blocks, odd, s);
+        else launch_probe<FLB_PROBE_INPLACE>(a, blocks, odd, s);
+      }
+    };
+    run(4);  // warm-up
+    hipEventRecord(e0, s);
+    run(launches);
+    hipEventRecord(e1, s);
+    float ms = 0.0f;
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = FLB_ERR_DEVICE;
+    *ms_per_launch = ms / float(launches);
+    if (bytes_per_launch) *bytes_per_launch = 72.0 * double(nx) * double(ny);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  (void)hipFree(a.f);
+  if (a.g) (void)hipFree(a.g);
+  return rc;
+}

@@ -118,8 +118,14 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
                const float* rho0, const float* u0, flb_handle** out);
 
 /* Free-surface fields after initFreeSurfaceFields (src/initData.h:175-206).
-   excess may be NULL (zero). */
+   excess may be NULL (zero).  The populations are not touched (nor, with
+   16-bit storage, the rounding residual stream-collide keeps for them), so
+   writing back the fields of flb_read / flb_read_fs leaves a run unchanged.
+   The _f64 form takes doubles: exact for FP64 storage (restoring an FP64
+   checkpoint), rounded to float otherwise. */
 int flb_set_fs_fields(flb_handle* h, const float* mass, const float* phi, const float* excess);
+int flb_set_fs_fields_f64(flb_handle* h, const double* mass, const double* phi,
+                          const double* excess);
 
 /* Advance nsteps lattice steps.  Replaces the step loops
    h_runCudaMonoFluidNoGraphics2D / h_runCudaFreeSurfaceNoGraphics2D
@@ -148,7 +154,11 @@ int flb_read_fs_f64(flb_handle* h, double* mass, double* excess);
    unshifted values (9N floats / doubles). */
 int flb_read_populations(flb_handle* h, float* f);
 int flb_read_populations_f64(flb_handle* h, double* f);
+/* Writes the raw population array (same layout).  With 16-bit storage the
+   stored values are rounded; a free-surface handle's rounding residual of the
+   replaced populations is cleared.  The _f64 form is exact for FP64 storage. */
 int flb_write_populations(flb_handle* h, const float* f);
+int flb_write_populations_f64(flb_handle* h, const double* f);
 
 /* Device pointers of the fields, for a HIP-GL interop consumer
    (the reference maps u onto a GL buffer, src/cuda/runCuda.cu:137-147).
@@ -172,6 +182,59 @@ uint64_t flb_time(const flb_handle* h);
 /* Grid size and problem type (FLB_MONOFLUID / FLB_FREE_SURFACE) of a handle;
    any output pointer may be NULL. */
 int flb_get_shape(const flb_handle* h, uint32_t* nx, uint32_t* ny, int32_t* problem);
+
+/* Viewer output (the GUI's device kernels).  Both enqueue on the handle's
+   stream and return; call flb_sync before the destination is read or its
+   GL resource unmapped (the reference synchronises the device once per frame,
+   src/cuda/runCuda.cu:258).  Destinations are device memory the caller owns,
+   e.g. a GL pixel buffer mapped with hipGraphicsMapResources /
+   hipGraphicsResourceGetMappedPointer and copied into the texture.
+
+   flb_export_velocity replaces CudaUtils::save2DDataToOpenGL
+   (src/cuda/cudaUtils.cu:107-118), launched per frame at
+   src/cuda/runCuda.cu:201-218: texel (x, y) of a float2 image with rows of
+   dst_pitch_bytes (>= 8 * nx, a multiple of 8, 8-byte aligned) receives
+   (u_x, u_y) of cell (x, y) -- no flip, as in the reference -- converted to
+   float for FP64 storage. */
+int flb_export_velocity(flb_handle* h, void* d_dst, size_t dst_pitch_bytes);
+
+/* flb_isosurface replaces d_CreateIsoSurfaceVector2D
+   (src/cuda/cudaIsosurface.cuh:33-198, with getInterpolatedPoint, drawLine and
+   marchingSquares, src/cuda/cudaIsosurface.cu:16-86), launched at
+   src/cuda/runCuda.cu:246-251 (velocity) and :418-426 (fill level).  Marching
+   squares over the squares of the camera window [x_min, x_max) x
+   [y_min, y_max), counted in display rows from the top (display row y spans
+   lattice rows ny-1-y and ny-2-y); each isoline segment is rasterised into
+   an 8-bit image of width x height pixels (rows of tex_pitch_bytes) by
+   Bresenham's line as 255, y flipped to GL's lower-left origin.  Vertex
+   values are |u| (FLB_ISO_VELOCITY) or the fill level phi (FLB_ISO_PHI,
+   free surface only; the reference passes iso_value 0.5 for it).  The
+   reference runs the velocity kernel on phi and so reads a second plane past
+   the end of phi (runCuda.cu:424); here phi is read as the scalar it is.
+   clear != 0 zeroes the image first (the reference leaves clearing to the
+   GL side).  Only the squares of the window are launched. */
+enum { FLB_ISO_VELOCITY = 0, FLB_ISO_PHI = 1 };
+typedef struct flb_iso_view {
+  uint32_t width, height;   /* texture size in pixels */
+  uint32_t x_min, x_max;    /* camera bounds in lattice nodes (getCameraDomainBounds) */
+  uint32_t y_min, y_max;
+  float iso_value;
+  int32_t field;            /* FLB_ISO_VELOCITY or FLB_ISO_PHI */
+  int32_t clear;
+} flb_iso_view;
+int flb_isosurface(flb_handle* h, const flb_iso_view* view, void* d_tex, size_t tex_pitch_bytes);
+
+/* Attainable-bandwidth reference for stream-collide (no reference
+   counterpart; the reference reports no bandwidth).  On a buffer of its own
+   (nine FP32 planes of an nx x ny grid, pitched as a handle's) on `device`,
+   times `launches` launches after four warm-ups and returns the mean device
+   time per launch; bytes_per_launch (may be NULL) receives the 72 bytes per
+   cell every kind moves.  FLB_PROBE_ESOTERIC: stream-collide's esoteric-pull
+   loads and stores in place with no arithmetic; FLB_PROBE_COPY: nine planes
+   copied to nine others; FLB_PROBE_INPLACE: nine planes incremented in place. */
+enum { FLB_PROBE_ESOTERIC = 0, FLB_PROBE_COPY = 1, FLB_PROBE_INPLACE = 2 };
+int flb_bandwidth_probe(int32_t device, uint32_t nx, uint32_t ny, int32_t kind, uint32_t launches,
+                        float* ms_per_launch, double* bytes_per_launch);
 
 /* Layout of the device fields of flb_device_fields: row stride `pitch`
    (cells, >= nx) and plane size `plane` (= pitch * ny elements).  No

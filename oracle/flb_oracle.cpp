@@ -837,6 +837,17 @@ void orc_set_f(void* s, const double* f) {
 double orc_total_mass(void* s) { return static_cast<Sim<double>*>(s)->total_mass(); }
 void orc_destroy(void* s) { delete static_cast<Sim<double>*>(s); }
 
+double orc_fs_discharge(void* s, uint32_t column) {
+  auto* q = static_cast<Sim<double>*>(s);
+  if (!q->p.free_surface) return 0.0;
+  double acc = 0.0;
+  for (uint32_t y = 0; y < q->ny; y++) {
+    const size_t n = column + size_t(y) * q->nx;
+    if (q->flags[n] & (FLUID | INTERFACE)) acc += q->phi[n] * q->rho[n] * q->u[n];
+  }
+  return acc;
+}
+
 void orc_set_cells(void* s, const uint8_t* flags, const double* rho, const double* u) {
   auto* q = static_cast<Sim<double>*>(s);
   if (flags) std::memcpy(q->flags.data(), flags, q->N);

@@ -76,6 +76,10 @@ void orc_set_f(void* s, const double* f);
 double orc_total_mass(void* s);
 void orc_destroy(void* s);
 
+/* Free-surface discharge through a column: sum of phi * rho * u_x over the
+   cells with a fluid or interface bit. */
+double orc_fs_discharge(void* s, uint32_t column);
+
 /* Single passes on a hand-built state, for the tests that pin each kernel.
    orc_set_cells overwrites flags / rho / u (each pointer may be NULL);
    orc_set_time sets the lattice time that selects the slot parity.

@@ -82,6 +82,8 @@ def lib():
         L.orc_get_reconstructed.argtypes = [P, dp, C.POINTER(C.c_uint16)]
         L.orc_discharge.argtypes = [P, C.c_uint32]
         L.orc_discharge.restype = C.c_double
+        L.orc_fs_discharge.argtypes = [P, C.c_uint32]
+        L.orc_fs_discharge.restype = C.c_double
         if hasattr(L, "orc_adjoint"):
             L.orc_adjoint.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64,
                                       C.c_uint32, dp, dp, dp]
@@ -210,6 +212,10 @@ class Oracle:
 
     def discharge(self, column):
         return lib().orc_discharge(self.h, column)
+
+    def fs_discharge(self, column):
+        """sum of phi rho u_x over the wet cells of a column (FP64 oracle)."""
+        return lib().orc_fs_discharge(self.h, column)
 
     def adjoint(self, nsteps, column, t0, t1, ckpt=100):
         g = np.zeros(self.N)

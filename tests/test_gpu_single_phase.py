@@ -1,7 +1,11 @@
 """GPU parity of the single-phase stream-collide kernel against the FP64 oracle,
 through the C ABI.  Tolerances: FP32 storage density and velocity within 1e-5
-relative L2 of the oracle after the full 512x128 x 10^4-step run; flags
-bit-exact; 16-bit storage formats within the bounds stated per test."""
+relative L2 of the oracle through t = 1000 of the 512x128 culvert (the FP64
+reference algorithm itself overflows on this configuration between t = 1750
+and t = 2000, so the 10^4-step run has no finite reference); at t = 1500 the
+FP32 error has grown to ~1.6e-5 with the flow (|u| up to 0.32) and is bounded
+by 1e-4; flags bit-exact; 16-bit storage formats within the bounds stated per
+test."""
 import numpy as np
 import pytest
 
@@ -53,9 +57,12 @@ def test_slots_match_after_few_steps(orc, steps):
 
 def test_parity_window_fp32(orc):
     """512 x 128 culvert: GPU FP32 (DDF-shifted) against the FP64 oracle over the
-    window in which the reference algorithm stays bounded (it diverges near
-    t = 1900 on this configuration, see test_reference_diverges).  The error
-    is also compared with the reference's own FP32 arithmetic (FP32 oracle)."""
+    window in which the reference algorithm stays bounded (it overflows between
+    t = 1750 and 2000 on this configuration, see
+    test_reference_diverges_on_512x128).  Bar: 1e-5 relative L2 in u and rho
+    through t = 1000; the t = 1500 value is printed and bounded by 1e-4 (the
+    rounding difference grows with the accelerating jet).  The error is also
+    compared with the reference's own FP32 arithmetic (FP32 oracle)."""
     flags, rho0, u0 = SP.build()
     s = SP.solver("fp32")
     o = oracle_for(orc, SP, flags, rho0, u0, threads=16)
@@ -71,26 +78,36 @@ def test_parity_window_fp32(orc):
         e32 = rel_l2(r32["u"], r["u"])
         print(f"\nfp32 512x128 t={t}: GPU relL2 u {e_u:.3e} rho {e_rho:.3e} | "
               f"reference-FP32 relL2 u {e32:.3e}")
-        assert e_u < 1e-4 and e_rho < 1e-5
-        if t <= 100:
-            assert e_u < 1e-5
+        if t <= 1000:
+            assert e_u < 1e-5 and e_rho < 1e-5
+        else:
+            assert e_u < 1e-4 and e_rho < 1e-5
         np.testing.assert_array_equal(g["flags"], flags)
     s.close()
 
 
 def test_reference_diverges_on_512x128(orc):
-    """The reference algorithm (FP64) on the 512 x 128 culvert is unstable: the
-    density rises above 1.2 and the run overflows before t = 2500."""
+    """The reference algorithm (FP64) on the 512 x 128 culvert is unstable: at
+    t = 1750 the jet through the culvert reaches |u| = 0.38 with rho in
+    [0.85, 1.28]; by t = 2000 the density is of order 1e37 and the run
+    overflows before t = 2500."""
     flags, rho0, u0 = SP.build()
     o = oracle_for(orc, SP, flags, rho0, u0, threads=16)
-    o.step(2500)
+    o.step(1750)
+    g = o.get()
+    act = (flags & 0x08) == 0
+    assert np.isfinite(g["rho"]).all() and g["rho"][act].max() < 1.3
+    o.step(250)
+    assert np.abs(o.get()["rho"][act]).max() > 1e6
+    o.step(500)
     assert not np.isfinite(o.get()["u"]).all()
 
 
-@pytest.mark.parametrize("storage,tol_u", [("fp16s", 2e-2), ("bf16s", 2e-1)])
+@pytest.mark.parametrize("storage,tol_u", [("fp16s", 5e-3), ("bf16s", 3e-2)])
 def test_sixteen_bit_storage(orc, storage, tol_u):
     """DDF-shifted 16-bit storage, FP32 arithmetic: velocity error vs FP64 after
-    1000 steps (reported; bounded loosely)."""
+    1000 steps.  Measured relative L2 of u: FP16S 5.3e-4, BF16S 5.5e-3; the
+    bars are about 10x and 5x those."""
     steps = 1000
     flags, rho0, u0 = SP.build()
     with SP.solver(storage) as s:
@@ -107,7 +124,7 @@ def test_sixteen_bit_storage(orc, storage, tol_u):
     print(f"\n{storage} 512x128 x {steps}: relL2 u {e:.3e} (fp32 {e32:.3e}) rho {rel_l2(g['rho'], r['rho']):.3e}")
     assert np.isfinite(g["u"]).all()
     assert e < tol_u
-    assert e32 < 1e-4
+    assert e32 < 1e-5
 
 
 @pytest.mark.parametrize("variant", [

@@ -129,31 +129,52 @@ def front(flags, nx, ny):
     return int(np.nonzero(wet.any(axis=0))[0].max())
 
 
+Q_COLUMN, Q_FROM, Q_EVERY = 3620, 1000, 100     # 8192 x 2048 discharge sampling
+
+
 def test_large_sixteen_bit_against_fp32():
+    """8192 x 2048, 3000 steps of the collapsing column: the storage formats
+    against FP32 on the front position, the surface heights, box-averaged
+    velocity and the time-averaged discharge through column 3620, which the
+    front crosses before step 1000 (mean over steps 1000-3000, every 100).
+    The barrel itself (x = 4000-4400) is reached only after ~4500 steps, when
+    this SRT run at tau = 0.51 has already turned to NaN.  Bars: FP16S within
+    0.5 % of FP32 on the discharge (the target for 16-bit storage); BF16S is
+    printed and held only to its measured levels."""
     case = cases.FS_8192
     nx, ny = case.nx, case.ny
     out = {}
     for st in ("fp32", "fp16s", "bf16s"):
+        qs = []
         with case.solver(st, mass_mode=1, gas_check_corrected=1) as s:
             m0 = s.mass()
-            s.step(BIG_STEPS)
-            out[st] = (s.read(), abs(s.mass() - m0) / m0)
-    g32, _ = out["fp32"]
+            s.step(Q_FROM)
+            for t in range(Q_FROM, BIG_STEPS + 1, Q_EVERY):
+                if t > Q_FROM:
+                    s.step(Q_EVERY)
+                g = s.read()
+                qs.append(discharge(g, g["flags"], nx, ny, Q_COLUMN))
+            out[st] = (g, abs(s.mass() - m0) / m0, float(np.mean(qs)))
+    g32, _, q32 = out["fp32"]
     ux32 = box(g32["u"][:nx * ny], nx, ny, 8)
     h32 = surface(g32["phi"], nx, ny).reshape(-1, 8).mean(axis=1)
     x32 = front(g32["flags"], nx, ny)
     assert x32 > case.water[0] + 100    # the column has collapsed
-    for st, (max_dx, max_mean_dh, max_du) in {"fp16s": (4, 0.5, 3e-2),
-                                              "bf16s": (24, 8.0, 0.4)}.items():
-        g, drift = out[st]
+    assert q32 > 1.0                    # water flows through the sampled column
+    for st, (max_dx, max_mean_dh, max_du, max_dq) in {"fp16s": (4, 0.5, 3e-2, 5e-3),
+                                                      "bf16s": (24, 8.0, 0.4, 0.1)}.items():
+        g, drift, q = out[st]
         ux = box(g["u"][:nx * ny], nx, ny, 8)
         h = surface(g["phi"], nx, ny).reshape(-1, 8).mean(axis=1)
         du = float(np.linalg.norm(ux - ux32) / np.linalg.norm(ux32))
         dh = np.abs(h - h32)
         x = front(g["flags"], nx, ny)
         dx = abs(x - x32)
+        dq = abs(q - q32) / abs(q32)
         print(f"\n{st} 8192x2048 x {BIG_STEPS} vs FP32: front {x} vs {x32}; "
               f"surface (8-column mean) max|dh| {dh.max():.3f} mean {dh.mean():.4f} cells; "
-              f"box-averaged relL2 u_x {du:.2e}; mass drift {drift:.2e} (fp32 {out['fp32'][1]:.2e})")
+              f"box-averaged relL2 u_x {du:.2e}; discharge at x = {Q_COLUMN} {q:.5e} vs "
+              f"{q32:.5e} (rel {dq:.2e}); mass drift {drift:.2e} (fp32 {out['fp32'][1]:.2e})")
         assert dx <= max_dx and dh.mean() <= max_mean_dh and du <= max_du
+        assert dq <= max_dq
         assert drift <= 1e-6
