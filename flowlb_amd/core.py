@@ -66,7 +66,8 @@ class Culvert(C.Structure):
 
 class AdjointSpec(C.Structure):
     _fields_ = [("nsteps", C.c_uint64), ("t0", C.c_uint64), ("t1", C.c_uint64),
-                ("column", C.c_uint32), ("ckpt", C.c_uint32)]
+                ("column", C.c_uint32), ("ckpt", C.c_uint32),
+                ("trace_budget_bytes", C.c_uint64)]
 
 
 class Instance(C.Structure):
@@ -496,9 +497,11 @@ class Solver:
         check(lib().flb_objective(self.h, C.byref(spec), C.byref(j)), self.h, "flb_objective")
         return j.value
 
-    def adjoint(self, nsteps, column, t0, t1, ckpt=100):
-        """Returns J, dJ/dalpha (N floats), dJ/du_inlet and the phase times (ms)."""
-        spec = AdjointSpec(nsteps, t0, t1, column, ckpt)
+    def adjoint(self, nsteps, column, t0, t1, ckpt=100, trace_budget_bytes=0):
+        """Returns J, dJ/dalpha (N floats), dJ/du_inlet and the phase times (ms):
+        forward sweep, recompute of the segments without a resident trace,
+        reverse sweep.  trace_budget_bytes: 0 = half the free device memory."""
+        spec = AdjointSpec(nsteps, t0, t1, column, ckpt, trace_budget_bytes)
         j = C.c_double()
         duin = C.c_double()
         g = np.zeros(self.N, np.float32)

@@ -12,9 +12,13 @@
 // rho and m = sum c_i f_i, so the forward recompute records three numbers per
 // cell and step (the trace) instead of nine populations.
 //
-// Memory: checkpoints of the populations and the inlet/outlet publication
-// buffers every `ckpt` steps; the trace of one checkpoint segment (3 N floats
-// per step); the adjoint population array (9 planes).
+// Memory: the trace of every step (3 N floats per step) is kept from the
+// forward sweep for as many of the last checkpoint segments as fit in the
+// trace budget (by default half the free device memory: on a 288 GB device
+// the whole 2000-step, 1024 x 256 rollout, 6.3 GB); the earlier segments get a
+// checkpoint of the populations and the inlet/outlet publication buffers every
+// `ckpt` steps and are recomputed, with their trace, during the reverse sweep.
+// The adjoint population array has 9 planes; dJ/dalpha accumulates in FP64.
 #include <algorithm>
 #include <vector>
 
@@ -27,7 +31,7 @@ struct AdjParams {
   float* abar;          // adjoint populations, 9 planes of Np floats
   float* rbar[2];       // adjoint of published densities, by parity
   float* ubar[2];       // adjoint of published velocities, by parity
-  float* galpha;        // dJ/dalpha accumulator (N) or nullptr
+  double* galpha;       // dJ/dalpha accumulator (N), FP64 over all steps, or nullptr
   const float* trace;   // (rho, m_x, m_y) or inlet/outlet reads of this step
   const float* rho_init;
   const float* u_init;
@@ -191,7 +195,7 @@ __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjP
         ab += -0.5f * sa * ubu;
 #pragma unroll
         for (int i = 0; i < 9; i++) gb[i] += rb + float(cx(i)) * mbx + float(cy(i)) * mby;
-        if (a.galpha) a.galpha[n] += ab;
+        if (a.galpha) a.galpha[n] += double(ab);
       }
 #pragma unroll
       for (int i = 0; i < 9; i++) a.abar[load_index<ODD>(p, n, nb, i)] = gb[i];
@@ -310,15 +314,18 @@ int check_adjoint_state(flb_handle* h, const flb_adjoint_spec* sp) {
   return FLB_OK;
 }
 
-// Forward steps [t, t+count) with the objective accumulated into d_j.
+// Forward steps [t, t+count) with the objective accumulated into d_j; with a
+// trace buffer, step t + k records its trace at trace + k * 3 N.
 int run_forward(flb_handle* h, uint64_t count, const flb_adjoint_spec* sp, double qscale,
-                double* d_j) {
+                double* d_j, float* trace = nullptr) {
   const int bx = flb::handle_block_x(h);
   for (uint64_t k = 0; k < count; ++k) {
     const uint64_t t = h->t;
     const bool win = d_j && t >= sp->t0 && t <= sp->t1;
     flb::Params p = flb::make_params(h, bx);
-    hipLaunchKernelGGL(flb::get_stream_collide(FLB_STORAGE_FP32, int(t & 1), false, win),
+    p.trace = trace ? trace + k * 3 * h->Nd : nullptr;
+    hipLaunchKernelGGL(trace ? flb::get_stream_collide_trace(int(t & 1), win)
+                             : flb::get_stream_collide(FLB_STORAGE_FP32, int(t & 1), false, win),
                        dim3(p.nblocks), dim3(bx), 0, h->stream, p);
     ADJ_HIP(h, hipGetLastError());
     if (win) {
@@ -382,17 +389,33 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
   const uint64_t K = sp->ckpt, T = sp->nsteps;
   const uint64_t nseg = (T + K - 1) / K;
   const size_t ck_floats = state_floats(h, false);
+  const size_t seg_bytes = size_t(std::min<uint64_t>(K, T)) * 3 * N * sizeof(float);
+  // Segments whose trace stays resident from the forward sweep: the last
+  // `resident` ones, as many as the budget holds.
+  size_t budget = sp->trace_budget_bytes;
+  if (budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    ADJ_HIP(h, hipMemGetInfo(&free_b, &total_b));
+    budget = free_b / 2;
+  }
+  const uint64_t resident = std::min<uint64_t>(nseg, budget / seg_bytes);
+  const uint64_t first_res = nseg - resident;  // segments [0, first_res) are recomputed
   DevBuf mem;
-  float* ckpt = mem.get<float>(nseg * ck_floats);
+  float* ckpt = first_res ? mem.get<float>(first_res * ck_floats) : nullptr;
   float* fields0 = mem.get<float>(state_floats(h, true));
-  float* trace = mem.get<float>(std::min<uint64_t>(K, T) * 3 * N);
+  float* trace_res = resident ? mem.get<float>(resident * (seg_bytes / sizeof(float))) : nullptr;
+  float* trace_tmp = first_res ? mem.get<float>(seg_bytes / sizeof(float)) : nullptr;
   float* abar = mem.get<float>(9 * Np);
   float* rbar = mem.get<float>(2 * N);
   float* ubar = mem.get<float>(4 * N);
-  float* galpha = h->alpha ? mem.get<float>(N) : nullptr;
+  double* galpha = h->alpha ? mem.get<double>(N) : nullptr;
   double* d_acc = mem.get<double>(2);  // J, dJ/du_in
-  if (!ckpt || !fields0 || !trace || !abar || !rbar || !ubar || (h->alpha && !galpha) || !d_acc)
+  if ((first_res && (!ckpt || !trace_tmp)) || (resident && !trace_res) || !fields0 || !abar ||
+      !rbar || !ubar || (h->alpha && !galpha) || !d_acc)
     return flb::set_error(h, FLB_ERR_MEMORY, "adjoint buffers (checkpoints, trace, adjoint state)");
+  auto seg_trace = [&](uint64_t s) {
+    return s >= first_res ? trace_res + (s - first_res) * (seg_bytes / sizeof(float)) : trace_tmp;
+  };
 
   if ((rc = copy_state(h, fields0, false, true))) return rc;
   ADJ_HIP(h, hipMemsetAsync(d_acc, 0, 2 * sizeof(double), h->stream));
@@ -400,11 +423,17 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
   for (auto& e : ev) ADJ_HIP(h, hipEventCreate(&e));
   ADJ_HIP(h, hipEventRecord(ev[0], h->stream));
 
-  // Forward sweep: checkpoints and the objective.
+  // Forward sweep: the objective, checkpoints of the segments to recompute,
+  // the trace of the resident ones.
   const double qs = 1.0 / double(sp->t1 - sp->t0 + 1);
   for (uint64_t s = 0; s < nseg && !rc; ++s) {
-    if ((rc = copy_state(h, ckpt + s * ck_floats, false, false))) break;
-    rc = run_forward(h, std::min(K, T - s * K), sp, qs, d_acc);
+    const uint64_t count = std::min(K, T - s * K);
+    if (s < first_res) {
+      if ((rc = copy_state(h, ckpt + s * ck_floats, false, false))) break;
+      rc = run_forward(h, count, sp, qs, d_acc);
+    } else {
+      rc = run_forward(h, count, sp, qs, d_acc, seg_trace(s));
+    }
   }
   ADJ_HIP(h, hipEventRecord(ev[1], h->stream));
 
@@ -412,24 +441,19 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
   ADJ_HIP(h, hipMemsetAsync(abar, 0, 9 * Np * sizeof(float), h->stream));
   ADJ_HIP(h, hipMemsetAsync(rbar, 0, 2 * N * sizeof(float), h->stream));
   ADJ_HIP(h, hipMemsetAsync(ubar, 0, 4 * N * sizeof(float), h->stream));
-  if (galpha) ADJ_HIP(h, hipMemsetAsync(galpha, 0, N * sizeof(float), h->stream));
-  const int bx = flb::handle_block_x(h);
+  if (galpha) ADJ_HIP(h, hipMemsetAsync(galpha, 0, N * sizeof(double), h->stream));
   const int abx = 256;
   float recompute_ms = 0.0f;
   for (uint64_t s = nseg; s-- > 0 && !rc;) {
     const uint64_t ts = s * K, te = std::min(ts + K, T);
-    if ((rc = copy_state(h, ckpt + s * ck_floats, true, false))) break;
-    h->t = ts;
-    ADJ_HIP(h, hipEventRecord(ev[2], h->stream));
-    for (uint64_t t = ts; t < te; ++t) {
-      flb::Params p = flb::make_params(h, bx);
-      p.trace = trace + (t - ts) * 3 * N;
-      hipLaunchKernelGGL(flb::get_stream_collide_trace(int(t & 1), false), dim3(p.nblocks), dim3(bx),
-                         0, h->stream, p);
-      ADJ_HIP(h, hipGetLastError());
-      h->t += 1;
+    float* trace = seg_trace(s);
+    if (s < first_res) {
+      if ((rc = copy_state(h, ckpt + s * ck_floats, true, false))) break;
+      h->t = ts;
+      ADJ_HIP(h, hipEventRecord(ev[2], h->stream));
+      if ((rc = run_forward(h, te - ts, sp, qs, nullptr, trace))) break;
+      ADJ_HIP(h, hipEventRecord(ev[3], h->stream));
     }
-    ADJ_HIP(h, hipEventRecord(ev[3], h->stream));
     for (uint64_t t = te; t-- > ts;) {
       flb::Params p = flb::make_params(h, abx);
       flb::AdjParams a{};
@@ -450,10 +474,12 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
         hipLaunchKernelGGL(flb::k_adjoint_step<0>, dim3(p.nblocks), dim3(abx), 0, h->stream, p, a);
       ADJ_HIP(h, hipGetLastError());
     }
-    ADJ_HIP(h, hipEventSynchronize(ev[3]));
-    float ms = 0.0f;
-    ADJ_HIP(h, hipEventElapsedTime(&ms, ev[2], ev[3]));
-    recompute_ms += ms;
+    if (s < first_res) {
+      ADJ_HIP(h, hipEventSynchronize(ev[3]));
+      float ms = 0.0f;
+      ADJ_HIP(h, hipEventElapsedTime(&ms, ev[2], ev[3]));
+      recompute_ms += ms;
+    }
   }
   if (!rc) {
     flb::Params p = flb::make_params(h, abx);
@@ -475,13 +501,17 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
   double acc[2] = {0.0, 0.0};
   ADJ_HIP(h, hipMemcpyAsync(acc, d_acc, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   ADJ_HIP(h, hipStreamSynchronize(h->stream));
-  if (dj_dalpha && galpha && !rc) rc = flb::copy_cells_out(h, dj_dalpha, galpha, sizeof(float), 1);
+  if (dj_dalpha && galpha && !rc) {
+    std::vector<double> g(h->N);
+    rc = flb::copy_cells_out(h, g.data(), galpha, sizeof(double), 1);
+    for (size_t n = 0; n < h->N && !rc; ++n) dj_dalpha[n] = float(g[n]);
+  }
   if (phase_ms) {
     float fwd = 0.0f, tot = 0.0f;
     ADJ_HIP(h, hipEventElapsedTime(&fwd, ev[0], ev[1]));
     ADJ_HIP(h, hipEventElapsedTime(&tot, ev[0], end));
-    phase_ms[0] = fwd;                   // forward sweep with checkpoints
-    phase_ms[1] = recompute_ms;          // forward recompute with trace
+    phase_ms[0] = fwd;                   // forward sweep (checkpoints or trace)
+    phase_ms[1] = recompute_ms;          // forward recompute of non-resident segments
     phase_ms[2] = tot - fwd - recompute_ms;  // reverse sweep and restore
   }
   for (auto& e : ev) (void)hipEventDestroy(e);

@@ -253,6 +253,7 @@ int flb_step_timed(flb_handle* h, uint64_t nsteps, float* total_ms, float* per_k
 /* Histogram of flags (256 counts), computed on the device. */
 int flb_flag_histogram(flb_handle* h, uint64_t* counts);
 
+
 /* Last error message of the handle ("" if none), and a name for a status. */
 const char* flb_last_error(const flb_handle* h);
 const char* flb_status_string(int status);
@@ -278,13 +279,19 @@ typedef struct flb_adjoint_spec {
   uint64_t t0, t1;
   uint32_t column;
   uint32_t ckpt;
+  /* Device bytes for per-step traces kept from the forward sweep (0: half the
+     free device memory).  Checkpoint segments whose trace does not fit are
+     recomputed from their checkpoint in the reverse sweep. */
+  uint64_t trace_budget_bytes;
 } flb_adjoint_spec;
 
 /* Forward objective only; the handle is left at t = 0. */
 int flb_objective(flb_handle* h, const flb_adjoint_spec* spec, double* J);
 
-/* Forward sweep with checkpoints, reverse sweep with recomputation of each
-   checkpoint segment.  Outputs J, dJ/dalpha (N floats, NULL allowed; zeros
+/* Forward sweep keeping the per-step trace of the last checkpoint segments
+   that fit in the trace budget and a checkpoint every ckpt steps for the
+   others; reverse sweep that recomputes only the segments without a resident
+   trace.  dJ/dalpha accumulates in FP64 and is returned rounded to float.  Outputs J, dJ/dalpha (N floats, NULL allowed; zeros
    without Brinkman drag), dJ/du_inlet (x component of the velocity held by
    the inlet cells).  phase_ms (NULL allowed) receives the device time of the
    forward sweep, the recompute passes and the reverse sweep.  The handle is

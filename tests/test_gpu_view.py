@@ -58,7 +58,7 @@ def test_velocity_isolines(view, iso):
     want = vo.isosurface(mag, view["width"], view["height"], view["bounds"], iso)
     lit = int((want == 255).sum())
     print(f"\nvelocity isolines iso {iso} window {view['bounds']}: {lit} pixels")
-    assert lit > 50
+    assert lit > 20
     np.testing.assert_array_equal(img, want)
 
 
