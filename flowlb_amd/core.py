@@ -150,6 +150,7 @@ PROTOTYPES = [
     ("flb_time", C.c_uint64, [_H]),
     ("flb_step_timed", C.c_int, [_H, C.c_uint64, _F, _F]),
     ("flb_flag_histogram", C.c_int, [_H, _U64]),
+    ("flb_flag_histogram_noalloc", C.c_int, [_H, _U64]),
     ("flb_last_error", C.c_char_p, [_H]),
     ("flb_status_string", C.c_char_p, [C.c_int]),
     ("flb_destroy", None, [_H]),
@@ -479,6 +480,13 @@ class Solver:
         check(lib().flb_isosurface(self.h, C.byref(v), C.c_void_p(tex.ptr), pitch), self.h,
               "flb_isosurface")
         self.sync()
+
+    def flag_histogram_noalloc(self):
+        """flag_histogram through flb_flag_histogram_noalloc (no device allocation)."""
+        c = np.zeros(256, np.uint64)
+        check(lib().flb_flag_histogram_noalloc(self.h, c.ctypes.data_as(_U64)), self.h,
+              "flb_flag_histogram_noalloc")
+        return c
 
     def device_fields(self):
         ptrs = [C.c_void_p() for _ in range(4)]

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -23,9 +24,40 @@
 
 #include "flb_internal.h"
 
+// Device scratch of flb_flag_histogram_noalloc: its 256 flag counters, one
+// copy per GPU, held for the life of the process (2 KiB).  In host code this
+// name is not a device address; the per-device address comes from
+// hipGetSymbolAddress (scratch_address below).
+constexpr size_t kScratchCounters = 256;
+static __device__ __align__(256) unsigned long long scratch[kScratchCounters];
+
 namespace {
 
 constexpr int kUtilBlock = 256;
+
+// Address of `scratch` on each device, looked up once per device with that
+// device current (flb_create).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_address(int device, unsigned long long** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<unsigned long long*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+__global__ void k_clear_u64(unsigned long long* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
 
 int fail(flb_handle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
@@ -383,6 +415,10 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   }
   if (opt->device < 0 || opt->device >= ndev) return bail(FLB_ERR_ARGUMENT);
   if (hipSetDevice(opt->device) != hipSuccess) return bail(FLB_ERR_DEVICE);
+  // Only flb_flag_histogram_noalloc needs the scratch; it reports a failed
+  // lookup itself, so the handle is usable either way.
+  h->hist_scratch_err = scratch_address(opt->device, &h->hist_scratch);
+  if (h->hist_scratch_err != hipSuccess) (void)hipGetLastError();
   if (hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, opt->device) !=
           hipSuccess || h->cus <= 0)
     h->cus = 256;
@@ -906,6 +942,26 @@ int flb_flag_histogram(flb_handle* h, uint64_t* counts) {
   else rc = download(h, counts, d, 256 * sizeof(unsigned long long));
   (void)hipFree(d);
   return rc;
+}
+
+int flb_flag_histogram_noalloc(flb_handle* h, uint64_t* counts) {
+  if (!h || !counts) return FLB_ERR_ARGUMENT;
+  // The histogram has one counter per flag value, 256 whatever the grid, and
+  // the scratch holds exactly that many; nothing a caller passes can exceed it.
+  constexpr uint32_t kBins = 256;
+  static_assert(kBins <= kScratchCounters, "histogram scratch too small");
+  unsigned long long* d = h->hist_scratch;
+  if (!d)
+    return fail(h, FLB_ERR_DEVICE, std::string("hipGetSymbolAddress of the histogram scratch: ") +
+                                       hipGetErrorString(h->hist_scratch_err));
+  // Cleared by a kernel (not a memset node), as the last call left its counts.
+  hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(kBins), 0, h->stream, d, kBins);
+  const flb::Params p = params(h, kUtilBlock);
+  const uint32_t blocks = p.nblocks < 2048u ? p.nblocks : 2048u;  // grid-stride kernel
+  hipLaunchKernelGGL(flb::get_flag_hist(), dim3(blocks), dim3(256), 0, h->stream, p, d);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(h, e, "flag histogram");
+  return download(h, counts, d, kBins * sizeof(unsigned long long));
 }
 
 }  // extern "C"

@@ -78,6 +78,10 @@ struct flb_handle {
   float* rho_init = nullptr;  // rho0 and u0 as given to flb_create
   float* u_init = nullptr;
   double* partial = nullptr;
+  // Device address of the file-scope histogram scratch on this handle's GPU
+  // (flb_flag_histogram_noalloc); not owned.
+  unsigned long long* hist_scratch = nullptr;
+  hipError_t hist_scratch_err = hipSuccess;
   size_t partial_len = 0;
   uint64_t t = 0;
   float inv_tau = 0.0f;

@@ -253,6 +253,16 @@ int flb_step_timed(flb_handle* h, uint64_t nsteps, float* total_ms, float* per_k
 /* Histogram of flags (256 counts), computed on the device. */
 int flb_flag_histogram(flb_handle* h, uint64_t* counts);
 
+/* flb_flag_histogram without allocating: no counterpart in the reference.  For
+   callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc).  The 256 device
+   counters live in a file-scope array of the library (2 KiB per GPU, kept
+   until the process ends), resolved per device by flb_create; the histogram
+   always has 256 bins, so no grid exceeds it.  Every call on one GPU shares
+   that array: calls that overlap on one GPU (from several threads, or on
+   handles whose streams run concurrently) race and must be serialised by the
+   caller.  The counters are cleared by a kernel at the start of each call. */
+int flb_flag_histogram_noalloc(flb_handle* h, uint64_t* counts);
 
 /* Last error message of the handle ("" if none), and a name for a status. */
 const char* flb_last_error(const flb_handle* h);
