@@ -101,8 +101,26 @@ def test_conservation_moving_front_1e4_steps(storage):
     assert drift <= 1e-7
 
 
+@pytest.mark.parametrize("storage", ["fp32", "fp16s"])
+def test_conservation_4096x1024_1e4_steps(storage):
+    """The 4096 x 1024 dam break (the reference's gas gate), 10^4 steps:
+    |M(t) - M(0)| / M(0) <= 1e-7 (measured: FP32 1.6e-11, FP16S 3.8e-10).
+    Without the FP32 collision's rounding folded into the rest population
+    the FP32 run drifted 3.7e-7."""
+    with cases.FS_4096.solver(storage, mass_mode=1) as s:
+        m0 = s.mass()
+        s.step(10000)
+        m1 = s.mass()
+        ok = np.isfinite(s.read()["u"]).all()
+    drift = abs(m1 - m0) / m0
+    print(f"\nfs 4096x1024 x 1e4 {storage} conservative: mass drift {drift:.3e}")
+    assert ok
+    assert drift <= 1e-7
+
+
 def test_conservation_fp32_1e4_steps():
-    """Conservative mode on the GPU: |M(t) - M(0)| / M(0) <= 1e-7 over 10^4 steps."""
+    """Conservative mode on the GPU, 512 x 128: |M(t) - M(0)| / M(0) over 10^4
+    steps, measured 8e-11; bar 1e-9 (the target is 1e-7)."""
     case = cases.scaled(cases.FS_4096, 8)      # 512 x 128
     with case.solver("fp32", mass_mode=1) as s:
         m0 = s.mass()
@@ -112,7 +130,7 @@ def test_conservation_fp32_1e4_steps():
     drift = abs(m1 - m0) / m0
     print(f"\nfs 512x128 x 1e4 fp32 conservative: mass drift {drift:.3e}")
     assert np.isfinite(g["u"]).all()
-    assert drift <= 1e-7
+    assert drift <= 1e-9
 
 
 @pytest.mark.parametrize("storage", ["fp16s", "bf16s"])
