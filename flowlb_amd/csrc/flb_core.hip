@@ -773,25 +773,47 @@ int flb_step_timed(flb_handle* h, uint64_t nsteps, float* total_ms, float* per_k
     (void)hipEventDestroy(b);
     return rc;
   }
-  hipEvent_t ev[2 * FLB_NUM_KERNELS];
-  for (auto& e : ev) FLB_HIP(h, hipEventCreate(&e));
+  // Per-kernel mode: every launch of every step between two events of its
+  // own, the steps enqueued as flb_step does (a free-surface handle waits for
+  // the step kInFlight back).  The events skip the system-scope release fence
+  // (hipEventDisableSystemFence): with it each record wrote back and
+  // invalidated the L2, and the next kernel started cold.
+  const bool fs = h->opt.problem == FLB_FREE_SURFACE;
+  const bool throttle = h->box_host != nullptr;
+  constexpr uint64_t K = flb_handle::kInFlight;
+  const size_t per = 2 * FLB_NUM_KERNELS;
+  std::vector<hipEvent_t> ev(per * nsteps, nullptr);
+  int rc = FLB_OK;
+  for (auto& e : ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
+      (void)hipGetLastError();
+      e = nullptr;
+      rc = fail(h, FLB_ERR_DEVICE, "hipEventCreateWithFlags failed");
+      break;
+    }
+  const int last_k = fs ? FLB_K_FS_UPDATE : FLB_K_STREAM_COLLIDE;
+  for (uint64_t s = 0; s < nsteps && !rc; ++s) {
+    if (throttle && s >= K && hipEventSynchronize(ev[per * (s - K) + 2 * last_k + 1]) != hipSuccess)
+      rc = fail(h, FLB_ERR_DEVICE, "hipEventSynchronize failed");
+    if (!rc) rc = enqueue_step(h, s + 1 == nsteps, ev.data() + per * s);
+  }
   for (int k = 0; k < FLB_NUM_KERNELS; ++k) per_kernel_ms[k] = 0.0f;
   float tot = 0.0f;
-  int rc = FLB_OK;
-  const bool fs = h->opt.problem == FLB_FREE_SURFACE;
-  for (uint64_t s = 0; s < nsteps && !rc; ++s) {
-    rc = enqueue_step(h, s + 1 == nsteps, ev);
-    if (rc) break;
-    FLB_HIP(h, hipEventSynchronize(ev[2 * (fs ? FLB_K_FS_UPDATE : FLB_K_STREAM_COLLIDE) + 1]));
+  if (!rc && nsteps && hipEventSynchronize(ev[per * (nsteps - 1) + 2 * last_k + 1]) != hipSuccess)
+    rc = fail(h, FLB_ERR_DEVICE, "hipEventSynchronize failed");
+  for (uint64_t s = 0; s < nsteps && !rc; ++s)
     for (int k = 0; k < FLB_NUM_KERNELS; ++k) {
       if (!fs && k != FLB_K_STREAM_COLLIDE) continue;
       float ms = 0.0f;
-      FLB_HIP(h, hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+      if (hipEventElapsedTime(&ms, ev[per * s + 2 * k], ev[per * s + 2 * k + 1]) != hipSuccess) {
+        rc = fail(h, FLB_ERR_DEVICE, "hipEventElapsedTime failed");
+        break;
+      }
       per_kernel_ms[k] += ms;
       tot += ms;
     }
-  }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto e : ev)
+    if (e) (void)hipEventDestroy(e);
   if (total_ms) *total_ms = tot;
   return rc;
 }
