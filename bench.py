@@ -311,13 +311,19 @@ def bench_batch(args):
            "ms_per_step": round(batched_ms, 5), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
            "config": {"workload": f"batch_{args.instances}", "instances": args.instances,
-                      "cells": ncells, "autotuned_block_x": bx, "autotuned_grid": grid,
-                      "graph_chunk": info["graph_chunk"], "tiles": info["tiles"]},
+                      "cells": ncells, "autotuned_block_x": bx,
+                      "autotuned_mode": "persistent" if grid < 0 else "launch per step",
+                      "autotuned_grid": (f"{-grid} resident blocks per CU" if grid < 0 else
+                                         ("one block per tile" if grid == 0 else f"{grid} blocks")),
+                      "autotuned_tile_order": info["order"],
+                      "steps_per_graph_or_cooperative_launch": info["graph_chunk"],
+                      "tiles": info["tiles"]},
            "one_at_a_time_ms_per_step": round(per_sync * 1e3, 4),
            "one_at_a_time_async_ms_per_step": round(per_async * 1e3, 4),
            "speedup_vs_one_at_a_time": round(per_sync * 1e3 / batched_ms, 3),
            "speedup_vs_one_at_a_time_async": round(per_async * 1e3 / batched_ms, 3),
-           "achieved_GBs": round(ncells * 73 / (batched_ms * 1e-3) / 1e9, 1)}
+           "achieved_GBs": round(ncells * 73 / (batched_ms * 1e-3) / 1e9, 1),
+           "one_at_a_time_async_GBs": round(ncells * 73 / (per_async * 1e3 * 1e-3) / 1e9, 1)}
     print(json.dumps(out), flush=True)
 
 

@@ -160,6 +160,9 @@ PROTOTYPES = [
     ("flb_batch_create", C.c_int, [C.POINTER(Instance), C.c_uint32, C.c_int32, C.c_int32,
                                    C.POINTER(_H)]),
     ("flb_batch_configure", C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32]),
+    ("flb_batch_configure_persistent", C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32]),
+    ("flb_batch_set_order", C.c_int, [_H, C.c_int32]),
+    ("flb_batch_order", C.c_int32, [_H]),
     ("flb_batch_autotune", C.c_int, [_H, C.c_uint32, C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32), _F]),
     ("flb_batch_step", C.c_int, [_H, C.c_uint64]),
@@ -573,6 +576,11 @@ class Batch:
     def configure(self, block_x=256, grid=2048, graph_chunk=100):
         self._check(lib().flb_batch_configure(self.h, block_x, grid, graph_chunk), "configure")
 
+    def configure_persistent(self, block_x=256, blocks_per_cu=2, chunk=100):
+        """One cooperative launch per `chunk` steps (grid barrier between steps)."""
+        self._check(lib().flb_batch_configure_persistent(self.h, block_x, blocks_per_cu, chunk),
+                    "configure_persistent")
+
     def autotune(self, steps=20):
         b, g, ms = C.c_int32(), C.c_int32(), C.c_float()
         self._check(lib().flb_batch_autotune(self.h, steps, C.byref(b), C.byref(g), C.byref(ms)),
@@ -607,7 +615,13 @@ class Batch:
         b, g, c, t = C.c_int32(), C.c_int32(), C.c_int32(), C.c_uint32()
         self._check(lib().flb_batch_info(self.h, C.byref(b), C.byref(g), C.byref(c), C.byref(t)),
                     "info")
-        return dict(block_x=b.value, grid=g.value, graph_chunk=c.value, tiles=t.value)
+        return dict(block_x=b.value, grid=g.value, graph_chunk=c.value, tiles=t.value,
+                    order=["rows", "instance", "bands"][lib().flb_batch_order(self.h)])
+
+    def set_order(self, order):
+        """Tile order of the packed list: "rows", "instance" or "bands"."""
+        self._check(lib().flb_batch_set_order(self.h, ["rows", "instance", "bands"].index(order)),
+                    "set_order")
 
     @property
     def time(self):

@@ -327,11 +327,28 @@ typedef struct flb_batch flb_batch;
 
 int flb_batch_create(const flb_instance* instances, uint32_t count, int32_t storage,
                      int32_t device, flb_batch** out);
-/* Threads per block (multiple of 64), blocks of the grid-stride launch (0 =
-   one block per tile), steps per captured graph (even; 0 = plain launches). */
+/* Launch modes of flb_batch_step. */
+enum { FLB_BATCH_LAUNCH = 0, FLB_BATCH_PERSISTENT = 1 };
+/* Order of the packed tile list: instances interleaved row by row (default),
+   instance after instance row-major, or instance after instance in bands of
+   32 rows walked column by column. */
+enum { FLB_BATCH_ORDER_ROWS = 0, FLB_BATCH_ORDER_INSTANCE = 1, FLB_BATCH_ORDER_BANDS = 2 };
+int flb_batch_set_order(flb_batch* b, int32_t order);
+/* FLB_BATCH_LAUNCH: threads per block (multiple of 64), blocks of the
+   grid-stride launch (0 = one block per tile), steps per captured graph (even;
+   0 = plain launches). */
 int flb_batch_configure(flb_batch* b, int32_t block_x, int32_t grid, int32_t graph_chunk);
-/* Times block sizes x grid sizes over `steps` steps each (the state advances)
-   and keeps the fastest. */
+/* FLB_BATCH_PERSISTENT: one cooperative launch runs up to `chunk` steps, its
+   blocks_per_cu resident blocks per CU walking the tile list and meeting at a
+   grid barrier between steps; the last step of each flb_batch_step call runs
+   as a plain launch and writes the readback fields.  Fails with
+   FLB_ERR_ARGUMENT when that many blocks cannot be resident together. */
+int flb_batch_configure_persistent(flb_batch* b, int32_t block_x, int32_t blocks_per_cu,
+                                   int32_t chunk);
+/* Times tile orders x block sizes x grid sizes of both modes over `steps`
+   steps each (the state advances) and keeps the fastest (flb_batch_order
+   reports the order).  best_grid: blocks of the launch
+   (0 = one per tile), or -(blocks per CU) when the persistent mode won. */
 int flb_batch_autotune(flb_batch* b, uint32_t steps, int32_t* best_block, int32_t* best_grid,
                        float* best_ms_per_step);
 int flb_batch_step(flb_batch* b, uint64_t nsteps);
@@ -341,8 +358,12 @@ int flb_batch_step_timed(flb_batch* b, uint64_t nsteps, float* ms);
 int flb_batch_read(flb_batch* b, uint32_t instance, float* u, float* rho);
 int flb_batch_read_populations(flb_batch* b, uint32_t instance, float* f);
 uint64_t flb_batch_time(const flb_batch* b);
+/* Current configuration: block size, grid (blocks of the launch, 0 = one per
+   tile; -(blocks per CU) in the persistent mode), steps per graph (per
+   cooperative launch in the persistent mode), tiles of the packed list. */
 int flb_batch_info(const flb_batch* b, int32_t* block_x, int32_t* grid, int32_t* graph_chunk,
                    uint32_t* ntiles);
+int32_t flb_batch_order(const flb_batch* b);
 const char* flb_batch_last_error(const flb_batch* b);
 void flb_batch_destroy(flb_batch* b);
 
