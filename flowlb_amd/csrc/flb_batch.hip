@@ -302,6 +302,8 @@ int flb_batch_create(const flb_instance* in, uint32_t count, int32_t storage, in
     if (!s.flags || s.nx < 2 || s.ny < 2 || !(s.tau > 0.5f)) return FLB_ERR_ARGUMENT;
     const size_t N = size_t(s.nx) * s.ny;
     if (s.flags[N - 1] & FLB_INLET) return FLB_ERR_ARGUMENT;
+    // Planes addressed through 32-bit offsets and buffer descriptors.
+    if (align_up(align_up(s.nx, 64) * s.ny, 64) * 4 >= (1ull << 32)) return FLB_ERR_ARGUMENT;
     if ((s.flags[0] & FLB_OUTLET) && !(s.flags[0] & FLB_INLET)) return FLB_ERR_ARGUMENT;
   }
   int ndev = 0;

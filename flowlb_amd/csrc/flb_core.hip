@@ -309,6 +309,12 @@ int validate(const flb_options* o, uint32_t nx, uint32_t ny, std::string& why) {
   if ((uint64_t(nx) + 128) * ny > 0xffffffffull / 2 ||
       uint64_t(o->row_pitch) * ny > 0xffffffffull / 2)
     return why = "grid too large for 32-bit cell indices", 1;
+  // Kernels address every device plane (populations and cell fields) through
+  // 32-bit byte offsets and buffer descriptors: a plane must stay under 4 GiB.
+  const uint64_t pitch = o->row_pitch ? uint64_t(o->row_pitch) : (uint64_t(nx) + 63) / 64 * 64;
+  const uint64_t plane = (pitch * ny + 63) / 64 * 64;
+  if (plane * 8 >= (1ull << 32))
+    return why = "grid too large: a device plane must stay under 4 GiB (pitch * ny < 2^29)", 1;
   return 0;
 }
 

@@ -34,6 +34,34 @@ __host__ __device__ constexpr R wgt_r(int i) {
   return i == 0 ? R(4) / R(9) : (i < 5 ? R(1) / R(9) : R(1) / R(36));
 }
 
+// Element i of an array through a 32-bit byte offset from its base.  With a
+// wavefront-uniform base the compiler emits the SGPR-base form of the memory
+// instruction (global_load/store ... saddr: 64-bit base in SGPRs, one VGPR of
+// offset) instead of a per-lane 64-bit address -- two VALU instructions and a
+// VGPR pair saved per access.  Every device plane is under 4 GiB (flb_create
+// and flb_batch_create check), so an in-plane offset cannot wrap.
+template <class T>
+__device__ __forceinline__ T& at(T* base, uint32_t i) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ const T& at(const T* base, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(T)));
+}
+
+// Buffer resource (V#) of one device plane of `bytes` bytes (< 4 GiB): the
+// memory instructions then take the plane's base and size in SGPRs and one
+// 32-bit VGPR offset per lane (buffer_load/store ... offen), and an offset
+// past the plane reads zero / writes nothing instead of faulting.  The third
+// configuration word is the gfx9-family value composable_kernel uses for
+// gfx950.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc plane_rsrc(const void* base, uint64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, int(uint32_t(bytes)),
+                                           0x00020000);
+}
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
 // ---------------------------------------------------------------- storage
 // All formats hold the DDF-shifted population f_i - w_i.  T is the stored
 // type, R the arithmetic type: FP32 for the 32- and 16-bit formats, FP64 for
@@ -43,8 +71,15 @@ struct StoreF32 {
   using T = float;
   using R = float;
   static constexpr int kBytes = 4;
-  __device__ static float load(const T* p, size_t i) { return p[i]; }
-  __device__ static void store(T* p, size_t i, float v) { p[i] = v; }
+  static constexpr bool kBuffer = true;
+  __device__ static float load(const T* p, uint32_t i) { return at(p, i); }
+  __device__ static void store(T* p, uint32_t i, float v) { at(p, i) = v; }
+  __device__ static float bload(Rsrc r, uint32_t i) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(i * 4u), 0, 0));
+  }
+  __device__ static void bstore(Rsrc r, uint32_t i, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, int(i * 4u), 0, 0);
+  }
   __device__ static float round(float v) { return v; }  // value a store keeps
 };
 
@@ -52,8 +87,15 @@ struct StoreF64 {
   using T = double;
   using R = double;
   static constexpr int kBytes = 8;
-  __device__ static double load(const T* p, size_t i) { return p[i]; }
-  __device__ static void store(T* p, size_t i, double v) { p[i] = v; }
+  static constexpr bool kBuffer = true;
+  __device__ static double load(const T* p, uint32_t i) { return at(p, i); }
+  __device__ static void store(T* p, uint32_t i, double v) { at(p, i) = v; }
+  __device__ static double bload(Rsrc r, uint32_t i) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(i * 8u), 0, 0));
+  }
+  __device__ static void bstore(Rsrc r, uint32_t i, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, int(i * 8u), 0, 0);
+  }
   __device__ static double round(double v) { return v; }
 };
 
@@ -63,8 +105,17 @@ struct StoreF16S {
   using T = _Float16;
   using R = float;
   static constexpr int kBytes = 2;
-  __device__ static float load(const T* p, size_t i) { return float(p[i]) * 3.0517578125e-05f; }
-  __device__ static void store(T* p, size_t i, float v) { p[i] = _Float16(v * 32768.0f); }
+  static constexpr bool kBuffer = true;
+  __device__ static float load(const T* p, uint32_t i) { return float(at(p, i)) * 3.0517578125e-05f; }
+  __device__ static void store(T* p, uint32_t i, float v) { at(p, i) = _Float16(v * 32768.0f); }
+  __device__ static float bload(Rsrc r, uint32_t i) {
+    const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(r, int(i * 2u), 0, 0);
+    return float(__builtin_bit_cast(_Float16, h)) * 3.0517578125e-05f;
+  }
+  __device__ static void bstore(Rsrc r, uint32_t i, float v) {
+    const _Float16 h = _Float16(v * 32768.0f);
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), r, int(i * 2u), 0, 0);
+  }
   __device__ static float round(float v) {
     return float(_Float16(v * 32768.0f)) * 3.0517578125e-05f;
   }
@@ -75,13 +126,23 @@ struct StoreBF16S {
   using T = uint16_t;
   using R = float;
   static constexpr int kBytes = 2;
-  __device__ static float load(const T* p, size_t i) {
-    return __uint_as_float(uint32_t(p[i]) << 16);
+  static constexpr bool kBuffer = true;
+  __device__ static float load(const T* p, uint32_t i) {
+    return __uint_as_float(uint32_t(at(p, i)) << 16);
   }
-  __device__ static void store(T* p, size_t i, float v) {
+  __device__ static void store(T* p, uint32_t i, float v) {
     uint32_t u = __float_as_uint(v);
     u += 0x7fffu + ((u >> 16) & 1u);
-    p[i] = uint16_t(u >> 16);
+    at(p, i) = uint16_t(u >> 16);
+  }
+  __device__ static float bload(Rsrc r, uint32_t i) {
+    const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(r, int(i * 2u), 0, 0);
+    return __uint_as_float(uint32_t(h) << 16);
+  }
+  __device__ static void bstore(Rsrc r, uint32_t i, float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(u >> 16), r, int(i * 2u), 0, 0);
   }
   __device__ static float round(float v) {
     uint32_t u = __float_as_uint(v);
