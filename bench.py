@@ -528,6 +528,12 @@ def main():
         ref.update(probes)
         if traffic:
             ref["stream_collide_measured_traffic_GBs"] = round(traffic / (r["sc_ms"] * 1e-3) / 1e9, 1)
+        # The probe of the same element size: stream-collide's algorithmic rate
+        # over what its memory pattern alone attains on this device.
+        pat = {"fp32": "esoteric_inplace_GBs", "fp16s": "esoteric_inplace16_GBs",
+               "bf16s": "esoteric_inplace16_GBs"}.get(args.storage)
+        if pat in probes:
+            ref["stream_collide_over_pattern_probe"] = round(achieved / probes[pat], 3)
         out["hbm_reference"] = ref
     if not args.no_sweep:
         sweep = {}

@@ -215,7 +215,7 @@ def hip():
     return _hip
 
 
-PROBE_KINDS = {"esoteric_inplace": 0, "copy9": 1, "inplace9": 2}
+PROBE_KINDS = {"esoteric_inplace": 0, "copy9": 1, "inplace9": 2, "esoteric_inplace16": 3}
 
 
 def bandwidth_probe(nx, ny, kind, launches=20, device=0):

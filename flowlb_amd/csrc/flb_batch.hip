@@ -303,7 +303,7 @@ int flb_batch_create(const flb_instance* in, uint32_t count, int32_t storage, in
     const size_t N = size_t(s.nx) * s.ny;
     if (s.flags[N - 1] & FLB_INLET) return FLB_ERR_ARGUMENT;
     // Planes addressed through 32-bit offsets and buffer descriptors.
-    if (align_up(align_up(s.nx, 64) * s.ny, 64) * 4 >= (1ull << 32)) return FLB_ERR_ARGUMENT;
+    if (flb::plane_stride(align_up(s.nx, 64) * s.ny, 4) * 4 >= (1ull << 32)) return FLB_ERR_ARGUMENT;
     if ((s.flags[0] & FLB_OUTLET) && !(s.flags[0] & FLB_INLET)) return FLB_ERR_ARGUMENT;
   }
   int ndev = 0;
@@ -330,11 +330,10 @@ int flb_batch_create(const flb_instance* in, uint32_t count, int32_t storage, in
   size_t tot_f = 0, tot_c = 0;
   for (uint32_t i = 0; i < count; ++i) {
     const size_t Nd = align_up(in[i].nx, 64) * in[i].ny;
-    const size_t Np = align_up(Nd, 64);
     off_f[i] = tot_f;
-    tot_f += 9 * Np;
+    tot_f += 9 * flb::plane_stride(Nd, uint32_t(eb));
     off_c[i] = tot_c;
-    tot_c += Np;
+    tot_c += align_up(Nd, 64);
   }
   uint8_t* f = nullptr;
   uint8_t* flags = nullptr;
@@ -362,7 +361,7 @@ int flb_batch_create(const flb_instance* in, uint32_t count, int32_t storage, in
     p.pitch = uint32_t(align_up(s.nx, 64));
     p.N = size_t(p.pitch) * s.ny;  // device plane of a cell field
     p.NL = N;                      // host plane
-    p.Np = align_up(p.N, 64);
+    p.Np = flb::plane_stride(p.N, uint32_t(eb));
     p.inv_tau = 1.0f / s.tau;
     p.use_force = s.use_force;
     p.fx = s.use_force ? s.fx : 0.0f;

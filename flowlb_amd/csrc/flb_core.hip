@@ -3,7 +3,8 @@
 // Device memory of one handle.  Every plane stores the nx x ny cells in rows
 // of `pitch` cells (Nd = pitch * ny elements per plane of a cell field):
 //   f          9 planes x Np elements of the storage type (Np = Nd rounded up
-//              to a multiple of 64 so every plane starts 256-byte aligned)
+//              to 8 KiB plus 4 KiB: flb::plane_stride; every plane starts
+//              256-byte aligned)
 //   flags[2]   Nd bytes each; free-surface pass 4 writes the second buffer
 //   rho, u     readback fields, 1 and 2 planes
 //   pub_rho[2], pub_u[2]   inlet/outlet neighbour values by step parity
@@ -312,8 +313,7 @@ int validate(const flb_options* o, uint32_t nx, uint32_t ny, std::string& why) {
   // Kernels address every device plane (populations and cell fields) through
   // 32-bit byte offsets and buffer descriptors: a plane must stay under 4 GiB.
   const uint64_t pitch = o->row_pitch ? uint64_t(o->row_pitch) : (uint64_t(nx) + 63) / 64 * 64;
-  const uint64_t plane = (pitch * ny + 63) / 64 * 64;
-  if (plane * 8 >= (1ull << 32))
+  if (flb::plane_stride(pitch * ny, 8) * 8 >= (1ull << 32))
     return why = "grid too large: a device plane must stay under 4 GiB (pitch * ny < 2^29)", 1;
   return 0;
 }
@@ -406,8 +406,8 @@ int flb_create(const flb_options* opt, const uint8_t* flags, uint32_t nx, uint32
   h->N = N;
   h->pitch = row_pitch(opt, nx);
   h->Nd = uint64_t(h->pitch) * ny;
-  h->Np = round_up(h->Nd, 64);
   h->elem_bytes = opt->storage == FLB_STORAGE_FP32 ? 4 : (opt->storage == FLB_STORAGE_FP64 ? 8 : 2);
+  h->Np = flb::plane_stride(h->Nd, uint32_t(h->elem_bytes));
   h->field_bytes = opt->storage == FLB_STORAGE_FP64 ? 8 : 4;
   h->inv_tau = 1.0f / opt->tau;
   auto bail = [&](int rc) {

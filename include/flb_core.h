@@ -226,13 +226,15 @@ int flb_isosurface(flb_handle* h, const flb_iso_view* view, void* d_tex, size_t 
 
 /* Attainable-bandwidth reference for stream-collide (no reference
    counterpart; the reference reports no bandwidth).  On a buffer of its own
-   (nine FP32 planes of an nx x ny grid, pitched as a handle's) on `device`,
-   times `launches` launches after four warm-ups and returns the mean device
-   time per launch; bytes_per_launch (may be NULL) receives the 72 bytes per
-   cell every kind moves.  FLB_PROBE_ESOTERIC: stream-collide's esoteric-pull
-   loads and stores in place with no arithmetic; FLB_PROBE_COPY: nine planes
-   copied to nine others; FLB_PROBE_INPLACE: nine planes incremented in place. */
-enum { FLB_PROBE_ESOTERIC = 0, FLB_PROBE_COPY = 1, FLB_PROBE_INPLACE = 2 };
+   (nine planes of an nx x ny grid, pitched and strided as a handle's) on
+   `device`, times `launches` launches after four warm-ups and returns the mean
+   device time per launch; bytes_per_launch (may be NULL) receives the bytes
+   the launch moves: 18 values per cell, 72 bytes in FP32, 36 in 16-bit.
+   FLB_PROBE_ESOTERIC: stream-collide's esoteric-pull loads and stores in place
+   in FP32 with no arithmetic; FLB_PROBE_ESOTERIC16: the same with 16-bit
+   values; FLB_PROBE_COPY: nine FP32 planes copied to nine others;
+   FLB_PROBE_INPLACE: nine FP32 planes incremented in place. */
+enum { FLB_PROBE_ESOTERIC = 0, FLB_PROBE_COPY = 1, FLB_PROBE_INPLACE = 2, FLB_PROBE_ESOTERIC16 = 3 };
 int flb_bandwidth_probe(int32_t device, uint32_t nx, uint32_t ny, int32_t kind, uint32_t launches,
                         float* ms_per_launch, double* bytes_per_launch);
 
