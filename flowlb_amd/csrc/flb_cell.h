@@ -7,6 +7,35 @@
 
 namespace flb {
 
+// Quotients a / d for one divisor d (the density of a cell: velocity, Guo's
+// half-force shift).  In FP32 each quotient is the IEEE-rounded a / d: the
+// refined reciprocal and the two correction steps are the sequence hipcc emits
+// for a float division, without its pre-scaling of operands near the ends of
+// the exponent range (none here: d is a density near 1, a is zero or far above
+// 2^-100) and with its final fix-up of zeros and specials; the reciprocal is
+// formed once per cell instead of once per division (22 -> 15 instructions
+// for the velocity, 33 -> 21 with the force).  FP64 divides.
+template <class R>
+struct Quotient {
+  R d;
+  __device__ explicit Quotient(R den) : d(den) {}
+  __device__ R operator()(R a) const { return a / d; }
+};
+template <>
+struct Quotient<float> {
+  float d, r;
+  __device__ explicit Quotient(float den) : d(den) {
+    const float r0 = __builtin_amdgcn_rcpf(den);
+    r = __builtin_fmaf(__builtin_fmaf(-den, r0, 1.0f), r0, r0);
+  }
+  __device__ float operator()(float a) const {
+    const float q0 = a * r;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-d, q0, a), r, q0);
+    const float q2 = __builtin_fmaf(__builtin_fmaf(-d, q1, a), r, q1);
+    return __builtin_amdgcn_div_fixupf(q2, d, a);
+  }
+};
+
 // Equilibrium minus weights (feq_i - w_i), calculateEquilibriumDDF2Q9
 // (src/cuda/cellCalculations.cuh:32-52) rearranged for shifted storage:
 // feq_i - w_i = w_i * ((rho - 1) + rho * A_i).  R is the arithmetic type.
@@ -322,7 +351,7 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     }
   }
   const uint8_t io = DENSE ? 0 : fl & (kInlet | kOutlet);
-  R rho, rho1, ux, uy;
+  R rho, rho1, ux, uy, mx = R(0), my = R(0);
   if (io & kInlet) {
     rho = at(prho1, lin.next);
     rho1 = rho - R(1);
@@ -348,15 +377,18 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
 #pragma unroll
     for (int i = 1; i < 9; i++) rho1 += g[i];
     rho = R(1) + rho1;
-    const R mx = g[1] - g[2] + g[5] - g[6] + g[7] - g[8];
-    const R my = g[3] - g[4] + g[5] - g[6] + g[8] - g[7];
-    ux = mx / rho;
-    uy = my / rho;
+    mx = g[1] - g[2] + g[5] - g[6] + g[7] - g[8];
+    my = g[3] - g[4] + g[5] - g[6] + g[8] - g[7];
     if (TRACE) {
       p.trace[n] = rho;
       p.trace[N + n] = mx;
       p.trace[2 * N + n] = my;
     }
+  }
+  const Quotient<R> over_rho(rho);
+  if (!io) {
+    ux = over_rho(mx);
+    uy = over_rho(my);
   }
   // A fluid cell whose eight neighbours are fluid or interface (general path).
   const bool inner = (FS && !DENSE) ? interior_fluid(fl, nf) : true;
@@ -397,14 +429,14 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     // for u (no reference counterpart; FP32 storage only).
     const R a = R(p.alpha[n]);
     const R gx0 = R(p.use_force ? p.fx : 0.0f), gy0 = R(p.use_force ? p.fy : 0.0f);
-    const R rhol2 = R(0.5f) / rho;
+    const R rhol2 = over_rho(R(0.5f));
     const R s = R(1) / (R(1) + R(0.5f) * a);
     ux = (ux + rhol2 * gx0) * s;
     uy = (uy + rhol2 * gy0) * s;
     guo_terms<R>(ux, uy, float(gx0 - a * rho * ux), float(gy0 - a * rho * uy), fi);
     forced = true;
   } else if (p.use_force) {
-    const R rhol2 = R(0.5f) / rho;
+    const R rhol2 = over_rho(R(0.5f));
     ux = ux + rhol2 * R(p.fx);
     uy = uy + rhol2 * R(p.fy);
     guo_terms<R>(ux, uy, p.fx, p.fy, fi);

@@ -121,33 +121,28 @@ struct StoreF16S {
   }
 };
 
-// bfloat16 (upper half of an FP32), round to nearest even on store.
+// bfloat16 (upper half of an FP32), round to nearest even on store: gfx950's
+// FP32 -> BF16 conversion (one instruction; for finite values the same bits as
+// adding 0x7fff plus the lowest kept bit and truncating).
 struct StoreBF16S {
   using T = uint16_t;
   using R = float;
   static constexpr int kBytes = 2;
   static constexpr bool kBuffer = true;
+  __device__ static unsigned short bits(float v) { return __builtin_bit_cast(unsigned short, __bf16(v)); }
   __device__ static float load(const T* p, uint32_t i) {
     return __uint_as_float(uint32_t(at(p, i)) << 16);
   }
-  __device__ static void store(T* p, uint32_t i, float v) {
-    uint32_t u = __float_as_uint(v);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    at(p, i) = uint16_t(u >> 16);
-  }
+  __device__ static void store(T* p, uint32_t i, float v) { at(p, i) = bits(v); }
   __device__ static float bload(Rsrc r, uint32_t i) {
     const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(r, int(i * 2u), 0, 0);
     return __uint_as_float(uint32_t(h) << 16);
   }
   __device__ static void bstore(Rsrc r, uint32_t i, float v) {
-    uint32_t u = __float_as_uint(v);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(u >> 16), r, int(i * 2u), 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b16(bits(v), r, int(i * 2u), 0, 0);
   }
   __device__ static float round(float v) {
-    uint32_t u = __float_as_uint(v);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return __uint_as_float(u & 0xffff0000u);
+    return __uint_as_float(uint32_t(bits(v)) << 16);
   }
 };
 

@@ -138,8 +138,9 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
       return;
     }
   } else if (e == kInterface) {
-    R rho, ux, uy;
-    if (fl & (kInlet | kOutlet)) {
+    R rho, ux, uy, mx = R(0), my = R(0);
+    const bool io = fl & (kInlet | kOutlet);
+    if (io) {
       rho = fld<R>(p.rho)[n];
       ux = fld<R>(p.u)[n];
       uy = fld<R>(p.u)[N + n];
@@ -148,14 +149,19 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
 #pragma unroll
       for (int i = 1; i < 9; i++) s += out[i];
       rho = R(1) + s;
-      ux = (out[1] - out[2] + out[5] - out[6] + out[7] - out[8]) / rho;
-      uy = (out[3] - out[4] + out[5] - out[6] + out[8] - out[7]) / rho;
+      mx = out[1] - out[2] + out[5] - out[6] + out[7] - out[8];
+      my = out[3] - out[4] + out[5] - out[6] + out[8] - out[7];
+    }
+    const Quotient<R> over_rho(rho);
+    if (!io) {
+      ux = over_rho(mx);
+      uy = over_rho(my);
     }
     R ph[9];
 #pragma unroll
     for (int i = 1; i < 9; i++) ph[i] = phi[nb[i]];
     const R phi0 = cons ? phi[n] : fill_level<R>(m, rho);
-    const R rhol2 = R(0.5f) / rho;
+    const R rhol2 = over_rho(R(0.5f));
     ux = ux + rhol2 * R(p.fx);
     uy = uy + rhol2 * R(p.fy);
     R fe[9];
