@@ -12,13 +12,18 @@
 // rho and m = sum c_i f_i, so the forward recompute records three numbers per
 // cell and step (the trace) instead of nine populations.
 //
-// Memory: the trace of every step (3 N floats per step) is kept from the
+// Memory: the trace of every step (3 N values per step) is kept from the
 // forward sweep for as many of the last checkpoint segments as fit in the
 // trace budget (by default half the free device memory: on a 288 GB device
 // the whole 2000-step, 1024 x 256 rollout, 6.3 GB); the earlier segments get a
 // checkpoint of the populations and the inlet/outlet publication buffers every
 // `ckpt` steps and are recomputed, with their trace, during the reverse sweep.
-// The adjoint population array has 9 planes; dJ/dalpha accumulates in FP64.
+// The adjoint state (9 population planes, the adjoints of the published
+// inlet/outlet values) and its arithmetic are FP64: over 2000 reverse steps an
+// FP32 adjoint lost about 2e-4 (relative) on cells with small gradients.  The
+// trace is what the forward computed: FP32 for an FP32 handle, FP64 for an
+// FP64 handle (FP64 forward and trace: the gradient of the FP64 discrete
+// model).  The forward's float constants enter as their exact double values.
 #include <algorithm>
 #include <vector>
 
@@ -28,28 +33,32 @@
 namespace flb {
 
 struct AdjParams {
-  float* abar;          // adjoint populations, 9 planes of Np floats
-  float* rbar[2];       // adjoint of published densities, by parity
-  float* ubar[2];       // adjoint of published velocities, by parity
+  double* abar;         // adjoint populations, 9 planes of Np
+  double* rbar[2];      // adjoint of published densities, by parity
+  double* ubar[2];      // adjoint of published velocities, by parity
   double* galpha;       // dJ/dalpha accumulator (N), FP64 over all steps, or nullptr
-  const float* trace;   // (rho, m_x, m_y) or inlet/outlet reads of this step
+  const void* trace;    // (rho, m_x, m_y) or inlet/outlet reads of this step, T
   const float* rho_init;
   const float* u_init;
   double* duin;         // dJ/du_inlet accumulator
   uint32_t column;
   int in_window;
-  float qbar;           // 1 / number of averaged steps
+  double qbar;          // 1 / number of averaged steps
 };
 
-__device__ __forceinline__ void feq_grad(float rho, float ux, float uy, float* fe, float* dfx,
-                                         float* dfy) {
-  const float usq = ux * ux + uy * uy;
+// Equilibrium and its velocity derivatives with the weights of the forward
+// arithmetic T (float weights for FP32, exact ones for FP64).
+template <class T>
+__device__ __forceinline__ void feq_grad(double rho, double ux, double uy, double* fe,
+                                         double* dfx, double* dfy) {
+  const double usq = ux * ux + uy * uy;
 #pragma unroll
   for (int i = 0; i < 9; i++) {
-    const float cu = float(cx(i)) * ux + float(cy(i)) * uy;
-    fe[i] = wgt(i) * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * usq);
-    dfx[i] = wgt(i) * rho * (3.0f * float(cx(i)) + 9.0f * cu * float(cx(i)) - 3.0f * ux);
-    dfy[i] = wgt(i) * rho * (3.0f * float(cy(i)) + 9.0f * cu * float(cy(i)) - 3.0f * uy);
+    const double w = double(wgt_r<T>(i)), ex = double(cx(i)), ey = double(cy(i));
+    const double cu = ex * ux + ey * uy;
+    fe[i] = w * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+    dfx[i] = w * rho * (3.0 * ex + 9.0 * cu * ex - 3.0 * ux);
+    dfy[i] = w * rho * (3.0 * ey + 9.0 * cu * ey - 3.0 * uy);
   }
 }
 
@@ -94,7 +103,7 @@ __device__ __forceinline__ void block_add(double v, double* target) {
   }
 }
 
-template <int ODD>
+template <class T, int ODD>
 __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjParams a) {
   uint32_t tx, y;
   block_tile(blockIdx.x, p.tiles_x, p.ny, tx, y);
@@ -109,7 +118,7 @@ __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjP
       nb[0] = n;
 #pragma unroll
       for (int i = 1; i < 9; i++) nb[i] = cell_neighbour(p, x, y, i);
-      float v[9];
+      double v[9];
 #pragma unroll
       for (int i = 0; i < 9; i++) v[i] = a.abar[store_index<ODD>(p, n, nb, i)];
       const Linear lin = linear_neighbours(p, x, y, n);
@@ -117,35 +126,36 @@ __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjP
       const uint8_t fl_next = p.flags[lin.next];
       const bool pub_r = lin.has_prev && (fl_prev & kInlet);
       const bool pub_v = lin.has_next && (fl_next & kOutlet) && !(fl & kInlet);
-      float rb = 0.0f, ubx = 0.0f, uby = 0.0f;
+      double rb = 0.0, ubx = 0.0, uby = 0.0;
       if (pub_r) {
         rb = a.rbar[ODD][n];
-        a.rbar[ODD][n] = 0.0f;
+        a.rbar[ODD][n] = 0.0;
       }
       if (pub_v) {
         ubx = a.ubar[ODD][n];
         uby = a.ubar[ODD][N + n];
-        a.ubar[ODD][n] = 0.0f;
-        a.ubar[ODD][N + n] = 0.0f;
+        a.ubar[ODD][n] = 0.0;
+        a.ubar[ODD][N + n] = 0.0;
       }
-      const float Fx = p.use_force ? p.fx : 0.0f, Fy = p.use_force ? p.fy : 0.0f;
-      const float t0 = a.trace[n], t1 = a.trace[N + n], t2 = a.trace[2 * N + n];
+      const double Fx = p.use_force ? p.fx : 0.0f, Fy = p.use_force ? p.fy : 0.0f;
+      const T* const tr = static_cast<const T*>(a.trace);
+      const double t0 = tr[n], t1 = tr[N + n], t2 = tr[2 * N + n];
       const uint8_t io = fl & (kInlet | kOutlet);
-      float gb[9];
+      double gb[9];
       if (io) {
         // populations = feq(rho, u_read + F/(2 rho)): no dependence on the loaded ones
-        const float rho = t0;
-        const float ux = t1 + 0.5f / rho * Fx, uy = t2 + 0.5f / rho * Fy;
-        float fe[9], dfx[9], dfy[9];
-        feq_grad(rho, ux, uy, fe, dfx, dfy);
+        const double rho = t0;
+        const double ux = t1 + 0.5 / rho * Fx, uy = t2 + 0.5 / rho * Fy;
+        double fe[9], dfx[9], dfy[9];
+        feq_grad<T>(rho, ux, uy, fe, dfx, dfy);
 #pragma unroll
         for (int i = 0; i < 9; i++) {
           rb += v[i] * fe[i] / rho;
           ubx += v[i] * dfx[i];
           uby += v[i] * dfy[i];
-          gb[i] = 0.0f;
+          gb[i] = 0.0;
         }
-        rb += -(ubx * Fx + uby * Fy) * 0.5f / (rho * rho);
+        rb += -(ubx * Fx + uby * Fy) * 0.5 / (rho * rho);
         if (io & kInlet) {
           a.rbar[ODD ^ 1][lin.next] += rb;
           duin = ubx;
@@ -154,48 +164,51 @@ __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjP
           a.ubar[ODD ^ 1][N + lin.prev] += uby;
         }
       } else {
-        const float rho = t0, mx = t1, my = t2;
-        const float al = p.alpha ? p.alpha[n] : 0.0f;
-        const float sa = 1.0f / (1.0f + 0.5f * al);
-        const float ux = (mx + 0.5f * Fx) * sa / rho, uy = (my + 0.5f * Fy) * sa / rho;
-        const float Gx = Fx - al * rho * ux, Gy = Fy - al * rho * uy;
+        const double rho = t0, mx = t1, my = t2;
+        const double al = p.alpha ? p.alpha[n] : 0.0f;
+        const double sa = 1.0 / (1.0 + 0.5 * al);
+        const double ux = (mx + 0.5 * Fx) * sa / rho, uy = (my + 0.5 * Fy) * sa / rho;
+        const double Gx = Fx - al * rho * ux, Gy = Fy - al * rho * uy;
         const bool forced = p.use_force || p.alpha;
-        const float om = p.inv_tau, tl = 1.0f - 0.5f * p.inv_tau, k3 = 0.33333334f;
-        float fe[9], dfx[9], dfy[9];
-        feq_grad(rho, ux, uy, fe, dfx, dfy);
-        float Gbx = 0.0f, Gby = 0.0f;
+        // The forward's SRT weights are float constants (also in FP64).
+        const double om = p.inv_tau, omc = 1.0f - p.inv_tau, tl = 1.0f - 0.5f * p.inv_tau;
+        const double k3 = double(0.33333334f);
+        double fe[9], dfx[9], dfy[9];
+        feq_grad<T>(rho, ux, uy, fe, dfx, dfy);
+        double Gbx = 0.0, Gby = 0.0;
 #pragma unroll
         for (int i = 0; i < 9; i++) {
-          gb[i] = (1.0f - om) * v[i];
-          const float feb = om * v[i];
+          const double ex = double(cx(i)), ey = double(cy(i));
+          gb[i] = omc * v[i];
+          const double feb = om * v[i];
           rb += feb * fe[i] / rho;
           ubx += feb * dfx[i];
           uby += feb * dfy[i];
           if (forced) {
-            const float Fb = tl * v[i] * 9.0f * wgt(i);
-            const float cG = float(cx(i)) * Gx + float(cy(i)) * Gy;
-            const float cu = float(cx(i)) * ux + float(cy(i)) * uy;
-            ubx += Fb * (cG * float(cx(i)) - k3 * Gx);
-            uby += Fb * (cG * float(cy(i)) - k3 * Gy);
-            Gbx += Fb * (float(cx(i)) * (cu + k3) - k3 * ux);
-            Gby += Fb * (float(cy(i)) * (cu + k3) - k3 * uy);
+            const double Fb = tl * v[i] * 9.0 * double(wgt_r<T>(i));
+            const double cG = ex * Gx + ey * Gy;
+            const double cu = ex * ux + ey * uy;
+            ubx += Fb * (cG * ex - k3 * Gx);
+            uby += Fb * (cG * ey - k3 * Gy);
+            Gbx += Fb * (ex * (cu + k3) - k3 * ux);
+            Gby += Fb * (ey * (cu + k3) - k3 * uy);
           }
         }
         ubx += -al * rho * Gbx;
         uby += -al * rho * Gby;
         rb += -al * (ux * Gbx + uy * Gby);
-        float ab = -rho * (ux * Gbx + uy * Gby);
+        double ab = -rho * (ux * Gbx + uy * Gby);
         if (a.in_window && x == a.column) {  // J += qbar * rho * u_x
           rb += a.qbar * ux;
           ubx += a.qbar * rho;
         }
-        const float mbx = ubx * sa / rho, mby = uby * sa / rho;
-        const float ubu = ubx * ux + uby * uy;
+        const double mbx = ubx * sa / rho, mby = uby * sa / rho;
+        const double ubu = ubx * ux + uby * uy;
         rb += -ubu / rho;
-        ab += -0.5f * sa * ubu;
+        ab += -0.5 * sa * ubu;
 #pragma unroll
-        for (int i = 0; i < 9; i++) gb[i] += rb + float(cx(i)) * mbx + float(cy(i)) * mby;
-        if (a.galpha) a.galpha[n] += double(ab);
+        for (int i = 0; i < 9; i++) gb[i] += rb + double(cx(i)) * mbx + double(cy(i)) * mby;
+        if (a.galpha) a.galpha[n] += ab;
       }
 #pragma unroll
       for (int i = 0; i < 9; i++) a.abar[load_index<ODD>(p, n, nb, i)] = gb[i];
@@ -205,7 +218,8 @@ __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjP
 }
 
 // The initial populations of inlet cells, feq(rho0, u0) stored with the odd
-// pattern, depend on the inlet velocity.
+// pattern, depend on the inlet velocity.  The set-up evaluates them in double
+// with exact weights for every storage format.
 __global__ void __launch_bounds__(256) k_adjoint_inlet_init(const Params p, const AdjParams a) {
   uint32_t tx, y;
   block_tile(blockIdx.x, p.tiles_x, p.ny, tx, y);
@@ -219,10 +233,10 @@ __global__ void __launch_bounds__(256) k_adjoint_inlet_init(const Params p, cons
       nb[0] = n;
 #pragma unroll
       for (int i = 1; i < 9; i++) nb[i] = cell_neighbour(p, x, y, i);
-      float fe[9], dfx[9], dfy[9];
-      feq_grad(a.rho_init[n], a.u_init[n], a.u_init[p.N + n], fe, dfx, dfy);
+      double fe[9], dfx[9], dfy[9];
+      feq_grad<double>(a.rho_init[n], a.u_init[n], a.u_init[p.N + n], fe, dfx, dfy);
 #pragma unroll
-      for (int i = 0; i < 9; i++) acc += double(a.abar[store_index<1>(p, n, nb, i)]) * dfx[i];
+      for (int i = 0; i < 9; i++) acc += a.abar[store_index<1>(p, n, nb, i)] * dfx[i];
     }
   }
   block_add(acc, a.duin);
@@ -230,13 +244,14 @@ __global__ void __launch_bounds__(256) k_adjoint_inlet_init(const Params p, cons
 
 // Discharge through a column: sum of rho * u_x over its non-wall cells, from the
 // readback fields of the last step, accumulated as scale * Q into acc[0].
+template <class R>
 __global__ void __launch_bounds__(256) k_discharge(const Params p, uint32_t column, double scale,
                                                    double* acc) {
   __shared__ double lds[4];
   double s = 0.0;
   for (uint32_t y = threadIdx.x; y < p.ny; y += blockDim.x) {
     const uint32_t n = column + y * p.pitch;
-    if (!(p.flags[n] & (kWall | kGas))) s += double(p.rho[n]) * double(p.u[n]);
+    if (!(p.flags[n] & (kWall | kGas))) s += double(fld<R>(p.rho)[n]) * double(fld<R>(p.u)[n]);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
@@ -275,23 +290,24 @@ struct DevBuf {
 
 // Copies the step state (populations and the inlet/outlet publication buffers,
 // plus the readback fields when with_fields) between the handle and buf.
-// Layout of buf (device planes of Nd cells): f (9 Np) | pub_rho[0] |
-// pub_rho[1] (N each) | pub_u[0] | pub_u[1] (2N each) | rho (N) | u (2N).
-size_t state_floats(const flb_handle* h, bool with_fields) {
-  return 9 * h->Np + 6 * h->Nd + (with_fields ? 3 * h->Nd : 0);
+// Layout of buf (device planes of Nd cells, bytes): f (9 Np populations) |
+// pub_rho[0] | pub_rho[1] (N each) | pub_u[0] | pub_u[1] (2N each) | rho (N) |
+// u (2N), the fields in the handle's field type.
+size_t state_bytes(const flb_handle* h, bool with_fields) {
+  return 9 * h->Np * h->elem_bytes + (6 * h->Nd + (with_fields ? 3 * h->Nd : 0)) * h->field_bytes;
 }
-int copy_state(flb_handle* h, float* buf, bool to_handle, bool with_fields) {
-  const size_t N = h->Nd, Np = h->Np;
+int copy_state(flb_handle* h, uint8_t* buf, bool to_handle, bool with_fields) {
+  const size_t N = h->Nd * h->field_bytes, o = 9 * h->Np * h->elem_bytes;
   struct Part {
     void* dev;
     size_t off, bytes;
-  } parts[7] = {{h->f, 0, 9 * Np * sizeof(float)},
-                {h->pub_rho[0], 9 * Np, N * sizeof(float)},
-                {h->pub_rho[1], 9 * Np + N, N * sizeof(float)},
-                {h->pub_u[0], 9 * Np + 2 * N, 2 * N * sizeof(float)},
-                {h->pub_u[1], 9 * Np + 4 * N, 2 * N * sizeof(float)},
-                {h->rho, 9 * Np + 6 * N, N * sizeof(float)},
-                {h->u, 9 * Np + 7 * N, 2 * N * sizeof(float)}};
+  } parts[7] = {{h->f, 0, o},
+                {h->pub_rho[0], o, N},
+                {h->pub_rho[1], o + N, N},
+                {h->pub_u[0], o + 2 * N, 2 * N},
+                {h->pub_u[1], o + 4 * N, 2 * N},
+                {h->rho, o + 6 * N, N},
+                {h->u, o + 7 * N, 2 * N}};
   const int count = with_fields ? 7 : 5;
   for (int k = 0; k < count; ++k) {
     void* a = to_handle ? parts[k].dev : static_cast<void*>(buf + parts[k].off);
@@ -304,9 +320,10 @@ int copy_state(flb_handle* h, float* buf, bool to_handle, bool with_fields) {
 int check_adjoint_state(flb_handle* h, const flb_adjoint_spec* sp) {
   if (!h || !sp) return FLB_ERR_ARGUMENT;
   if (h->opt.problem != FLB_MONOFLUID || h->opt.collision != FLB_SRT || h->opt.smagorinsky ||
-      h->opt.storage != FLB_STORAGE_FP32)
+      (h->opt.storage != FLB_STORAGE_FP32 && h->opt.storage != FLB_STORAGE_FP64))
     return flb::set_error(h, FLB_ERR_UNSUPPORTED,
-                          "adjoint needs a monofluid SRT handle with FP32 storage and no subgrid model");
+                          "adjoint needs a monofluid SRT handle with FP32 or FP64 storage and no "
+                          "subgrid model");
   if (h->t != 0) return flb::set_error(h, FLB_ERR_STATE, "adjoint runs from the initial state (t = 0)");
   if (sp->nsteps == 0 || sp->t1 < sp->t0 || sp->t1 >= sp->nsteps || sp->column >= h->nx ||
       sp->ckpt == 0)
@@ -315,22 +332,23 @@ int check_adjoint_state(flb_handle* h, const flb_adjoint_spec* sp) {
 }
 
 // Forward steps [t, t+count) with the objective accumulated into d_j; with a
-// trace buffer, step t + k records its trace at trace + k * 3 N.
+// trace buffer, step t + k records its trace at trace + k * 3 N values.
 int run_forward(flb_handle* h, uint64_t count, const flb_adjoint_spec* sp, double qscale,
-                double* d_j, float* trace = nullptr) {
+                double* d_j, uint8_t* trace = nullptr) {
+  const bool fp64 = h->opt.storage == FLB_STORAGE_FP64;
   const int bx = flb::handle_block_x(h);
   for (uint64_t k = 0; k < count; ++k) {
     const uint64_t t = h->t;
     const bool win = d_j && t >= sp->t0 && t <= sp->t1;
     flb::Params p = flb::make_params(h, bx);
-    p.trace = trace ? trace + k * 3 * h->Nd : nullptr;
-    hipLaunchKernelGGL(trace ? flb::get_stream_collide_trace(int(t & 1), win)
-                             : flb::get_stream_collide(FLB_STORAGE_FP32, int(t & 1), false, win),
+    p.trace = trace ? reinterpret_cast<float*>(trace + k * 3 * h->Nd * h->field_bytes) : nullptr;
+    hipLaunchKernelGGL(trace ? flb::get_stream_collide_trace(h->opt.storage, int(t & 1), win)
+                             : flb::get_stream_collide(h->opt.storage, int(t & 1), false, win),
                        dim3(p.nblocks), dim3(bx), 0, h->stream, p);
     ADJ_HIP(h, hipGetLastError());
     if (win) {
-      hipLaunchKernelGGL(flb::k_discharge, dim3(1), dim3(256), 0, h->stream, p, sp->column, qscale,
-                         d_j);
+      hipLaunchKernelGGL(fp64 ? flb::k_discharge<double> : flb::k_discharge<float>, dim3(1),
+                         dim3(256), 0, h->stream, p, sp->column, qscale, d_j);
       ADJ_HIP(h, hipGetLastError());
     }
     h->t += 1;
@@ -364,7 +382,7 @@ int flb_objective(flb_handle* h, const flb_adjoint_spec* sp, double* J) {
   int rc = check_adjoint_state(h, sp);
   if (rc) return rc;
   DevBuf mem;
-  float* saved = mem.get<float>(state_floats(h, true));
+  uint8_t* saved = mem.get<uint8_t>(state_bytes(h, true));
   double* d_j = mem.get<double>(1);
   if (!saved || !d_j) return flb::set_error(h, FLB_ERR_MEMORY, "objective buffers");
   if ((rc = copy_state(h, saved, false, true))) return rc;
@@ -388,8 +406,9 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
   const size_t N = h->Nd, Np = h->Np;  // device planes
   const uint64_t K = sp->ckpt, T = sp->nsteps;
   const uint64_t nseg = (T + K - 1) / K;
-  const size_t ck_floats = state_floats(h, false);
-  const size_t seg_bytes = size_t(std::min<uint64_t>(K, T)) * 3 * N * sizeof(float);
+  const size_t ck_bytes = state_bytes(h, false);
+  const size_t seg_bytes = size_t(std::min<uint64_t>(K, T)) * 3 * N * h->field_bytes;
+  const bool fp64 = h->opt.storage == FLB_STORAGE_FP64;
   // Segments whose trace stays resident from the forward sweep: the last
   // `resident` ones, as many as the budget holds.
   size_t budget = sp->trace_budget_bytes;
@@ -401,20 +420,20 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
   const uint64_t resident = std::min<uint64_t>(nseg, budget / seg_bytes);
   const uint64_t first_res = nseg - resident;  // segments [0, first_res) are recomputed
   DevBuf mem;
-  float* ckpt = first_res ? mem.get<float>(first_res * ck_floats) : nullptr;
-  float* fields0 = mem.get<float>(state_floats(h, true));
-  float* trace_res = resident ? mem.get<float>(resident * (seg_bytes / sizeof(float))) : nullptr;
-  float* trace_tmp = first_res ? mem.get<float>(seg_bytes / sizeof(float)) : nullptr;
-  float* abar = mem.get<float>(9 * Np);
-  float* rbar = mem.get<float>(2 * N);
-  float* ubar = mem.get<float>(4 * N);
+  uint8_t* ckpt = first_res ? mem.get<uint8_t>(first_res * ck_bytes) : nullptr;
+  uint8_t* fields0 = mem.get<uint8_t>(state_bytes(h, true));
+  uint8_t* trace_res = resident ? mem.get<uint8_t>(resident * seg_bytes) : nullptr;
+  uint8_t* trace_tmp = first_res ? mem.get<uint8_t>(seg_bytes) : nullptr;
+  double* abar = mem.get<double>(9 * Np);
+  double* rbar = mem.get<double>(2 * N);
+  double* ubar = mem.get<double>(4 * N);
   double* galpha = h->alpha ? mem.get<double>(N) : nullptr;
   double* d_acc = mem.get<double>(2);  // J, dJ/du_in
   if ((first_res && (!ckpt || !trace_tmp)) || (resident && !trace_res) || !fields0 || !abar ||
       !rbar || !ubar || (h->alpha && !galpha) || !d_acc)
     return flb::set_error(h, FLB_ERR_MEMORY, "adjoint buffers (checkpoints, trace, adjoint state)");
   auto seg_trace = [&](uint64_t s) {
-    return s >= first_res ? trace_res + (s - first_res) * (seg_bytes / sizeof(float)) : trace_tmp;
+    return s >= first_res ? trace_res + (s - first_res) * seg_bytes : trace_tmp;
   };
 
   if ((rc = copy_state(h, fields0, false, true))) return rc;
@@ -429,7 +448,7 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
   for (uint64_t s = 0; s < nseg && !rc; ++s) {
     const uint64_t count = std::min(K, T - s * K);
     if (s < first_res) {
-      if ((rc = copy_state(h, ckpt + s * ck_floats, false, false))) break;
+      if ((rc = copy_state(h, ckpt + s * ck_bytes, false, false))) break;
       rc = run_forward(h, count, sp, qs, d_acc);
     } else {
       rc = run_forward(h, count, sp, qs, d_acc, seg_trace(s));
@@ -438,17 +457,17 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
   ADJ_HIP(h, hipEventRecord(ev[1], h->stream));
 
   // Reverse sweep, one checkpoint segment at a time.
-  ADJ_HIP(h, hipMemsetAsync(abar, 0, 9 * Np * sizeof(float), h->stream));
-  ADJ_HIP(h, hipMemsetAsync(rbar, 0, 2 * N * sizeof(float), h->stream));
-  ADJ_HIP(h, hipMemsetAsync(ubar, 0, 4 * N * sizeof(float), h->stream));
+  ADJ_HIP(h, hipMemsetAsync(abar, 0, 9 * Np * sizeof(double), h->stream));
+  ADJ_HIP(h, hipMemsetAsync(rbar, 0, 2 * N * sizeof(double), h->stream));
+  ADJ_HIP(h, hipMemsetAsync(ubar, 0, 4 * N * sizeof(double), h->stream));
   if (galpha) ADJ_HIP(h, hipMemsetAsync(galpha, 0, N * sizeof(double), h->stream));
   const int abx = 256;
   float recompute_ms = 0.0f;
   for (uint64_t s = nseg; s-- > 0 && !rc;) {
     const uint64_t ts = s * K, te = std::min(ts + K, T);
-    float* trace = seg_trace(s);
+    uint8_t* trace = seg_trace(s);
     if (s < first_res) {
-      if ((rc = copy_state(h, ckpt + s * ck_floats, true, false))) break;
+      if ((rc = copy_state(h, ckpt + s * ck_bytes, true, false))) break;
       h->t = ts;
       ADJ_HIP(h, hipEventRecord(ev[2], h->stream));
       if ((rc = run_forward(h, te - ts, sp, qs, nullptr, trace))) break;
@@ -463,15 +482,14 @@ int flb_adjoint(flb_handle* h, const flb_adjoint_spec* sp, double* J, float* dj_
       a.ubar[0] = ubar;
       a.ubar[1] = ubar + 2 * N;
       a.galpha = galpha;
-      a.trace = trace + (t - ts) * 3 * N;
+      a.trace = trace + (t - ts) * 3 * N * h->field_bytes;
       a.duin = d_acc + 1;
       a.column = sp->column;
       a.in_window = t >= sp->t0 && t <= sp->t1;
-      a.qbar = float(qs);
-      if (t & 1)
-        hipLaunchKernelGGL(flb::k_adjoint_step<1>, dim3(p.nblocks), dim3(abx), 0, h->stream, p, a);
-      else
-        hipLaunchKernelGGL(flb::k_adjoint_step<0>, dim3(p.nblocks), dim3(abx), 0, h->stream, p, a);
+      a.qbar = qs;
+      const auto kernel = fp64 ? ((t & 1) ? flb::k_adjoint_step<double, 1> : flb::k_adjoint_step<double, 0>)
+                               : ((t & 1) ? flb::k_adjoint_step<float, 1> : flb::k_adjoint_step<float, 0>);
+      hipLaunchKernelGGL(kernel, dim3(p.nblocks), dim3(abx), 0, h->stream, p, a);
       ADJ_HIP(h, hipGetLastError());
     }
     if (s < first_res) {

@@ -358,9 +358,9 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     ux = at(u_f, n);
     uy = at(u_f + N, n);
     if (TRACE) {
-      p.trace[n] = rho;
-      p.trace[N + n] = ux;
-      p.trace[2 * N + n] = uy;
+      fld<R>(p.trace)[n] = rho;
+      fld<R>(p.trace)[N + n] = ux;
+      fld<R>(p.trace)[2 * N + n] = uy;
     }
   } else if (io & kOutlet) {
     rho = at(rho_f, n);
@@ -368,9 +368,9 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     ux = at(pu1, lin.prev);
     uy = at(pu1 + N, lin.prev);
     if (TRACE) {
-      p.trace[n] = rho;
-      p.trace[N + n] = ux;
-      p.trace[2 * N + n] = uy;
+      fld<R>(p.trace)[n] = rho;
+      fld<R>(p.trace)[N + n] = ux;
+      fld<R>(p.trace)[2 * N + n] = uy;
     }
   } else {
     rho1 = g[0];
@@ -380,9 +380,9 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     mx = g[1] - g[2] + g[5] - g[6] + g[7] - g[8];
     my = g[3] - g[4] + g[5] - g[6] + g[8] - g[7];
     if (TRACE) {
-      p.trace[n] = rho;
-      p.trace[N + n] = mx;
-      p.trace[2 * N + n] = my;
+      fld<R>(p.trace)[n] = rho;
+      fld<R>(p.trace)[N + n] = mx;
+      fld<R>(p.trace)[2 * N + n] = my;
     }
   }
   const Quotient<R> over_rho(rho);

@@ -184,7 +184,7 @@ struct Params {
   float* phi;
   float* rec;            // reconstructed gas populations, 9 planes of N
   const float* alpha;    // Brinkman solid fraction or nullptr
-  float* trace;          // adjoint trace of one step: 3 planes of N (or nullptr)
+  float* trace;          // adjoint trace of one step: 3 planes of N elements of R (or nullptr)
   // Free surface: one word per kFsTileW x kFsTileH tile (work_x per tile row)
   // telling which of its 64-cell row chunks have work next step: bit r for
   // stream-collide in row r of the tile, bit 8 + r for pass 1.  Written by the

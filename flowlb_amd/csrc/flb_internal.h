@@ -35,7 +35,7 @@ KernelFn get_stream_collide_surface(int storage, int odd, bool mom);
 KernelFn get_stream_collide_all(int storage, int odd, bool mom);  // nullptr for FP64
 KernelFn get_fs1(int storage, int odd);
 KernelFn get_fs_update(int storage, int odd, uint32_t nx, uint32_t ny);
-KernelFn get_stream_collide_trace(int odd, bool mom);
+KernelFn get_stream_collide_trace(int storage, int odd, bool mom);  // FP32 or FP64
 MassFn get_pop_mass(int storage);
 MassFn get_fs_mass(int storage);
 ConvOutFn get_pop_to_f32(int storage);

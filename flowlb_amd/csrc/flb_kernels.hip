@@ -963,7 +963,13 @@ KernelFn get_fs_update(int storage, int odd, uint32_t nx, uint32_t ny) {
     default: return fs_update_fn<StoreF32>(odd, small);
   }
 }
-KernelFn get_stream_collide_trace(int odd, bool mom) {
+KernelFn get_stream_collide_trace(int storage, int odd, bool mom) {
+  if (storage == 3) {  // FP64
+    if (mom) return odd ? k_stream_collide<StoreF64, 1, false, true, true>
+                        : k_stream_collide<StoreF64, 0, false, true, true>;
+    return odd ? k_stream_collide<StoreF64, 1, false, false, true>
+               : k_stream_collide<StoreF64, 0, false, false, true>;
+  }
   if (mom) return odd ? k_stream_collide<StoreF32, 1, false, true, true>
                       : k_stream_collide<StoreF32, 0, false, true, true>;
   return odd ? k_stream_collide<StoreF32, 1, false, false, true>

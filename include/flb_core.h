@@ -276,7 +276,10 @@ void flb_destroy(flb_handle* h);
 /* ------------------------------------------------------------ adjoint */
 /* Gradients of a time-averaged discharge.  No reference counterpart (FlowLB
    has no adjoint); these entry points extend the solver for optimisation of a
-   culvert inlet.  Monofluid, SRT, FP32 storage, no subgrid model. */
+   culvert inlet.  Monofluid, SRT, FP32 or FP64 storage, no subgrid model.
+   With FP64 storage the forward sweep, its trace and the adjoint are FP64
+   throughout (the gradient of the FP64 discrete model); with FP32 storage the
+   forward and its trace are FP32 and the adjoint FP64. */
 
 /* Brinkman drag: per-cell solid fraction alpha (N floats) adds the force
    -alpha*rho*u to the Guo forcing of non-inlet/outlet cells, with the half-force
@@ -303,7 +306,8 @@ int flb_objective(flb_handle* h, const flb_adjoint_spec* spec, double* J);
 /* Forward sweep keeping the per-step trace of the last checkpoint segments
    that fit in the trace budget and a checkpoint every ckpt steps for the
    others; reverse sweep that recomputes only the segments without a resident
-   trace.  dJ/dalpha accumulates in FP64 and is returned rounded to float.  Outputs J, dJ/dalpha (N floats, NULL allowed; zeros
+   trace.  The adjoint state and dJ/dalpha are FP64; dJ/dalpha is returned
+   rounded to float.  Outputs J, dJ/dalpha (N floats, NULL allowed; zeros
    without Brinkman drag), dJ/du_inlet (x component of the velocity held by
    the inlet cells).  phase_ms (NULL allowed) receives the device time of the
    forward sweep, the recompute passes and the reverse sweep.  The handle is

@@ -51,6 +51,31 @@ def test_adjoint_matches_fp64_adjoint(orc):
     assert abs(duin - duinr) <= 1e-4 * abs(duinr)
 
 
+def test_fp64_adjoint_matches_fp64_oracle_adjoint(orc):
+    """An FP64 handle runs the forward sweep, its trace and the adjoint in
+    FP64: the gradient of the same discrete model the FP64 oracle adjoint
+    differentiates, so the two agree to rounding (dJ/dalpha is returned as
+    float)."""
+    c, flags, rho0, u0, alpha, win = small_case()
+    col = c.nx - 2
+    with core.Solver(flags, c.nx, c.ny, c.options(storage="fp64"), rho0=rho0, u0=u0) as s:
+        s.set_brinkman(alpha.astype(np.float32))
+        J, g, duin, ph = s.adjoint(STEPS, col, T0, T1, ckpt=50)
+    o = orc.Oracle(flags, c.nx, c.ny, c.tau, use_force=True, fx=c.force[0], fy=c.force[1],
+                   rho0=rho0.astype(np.float64), u0=u0.astype(np.float64),
+                   alpha=alpha.astype(np.float32).astype(np.float64), threads=8)
+    Jr, gr, duinr = o.adjoint(STEPS, col, T0, T1, ckpt=50)
+    m = alpha != 0
+    eg = np.linalg.norm(g[m] - gr[m]) / np.linalg.norm(gr[m])
+    ej, eu = abs(J - Jr) / abs(Jr), abs(duin - duinr) / abs(duinr)
+    print(f"\nFP64 adjoint 128x32: J rel {ej:.2e}; dJ/dalpha relL2 window {eg:.2e}; "
+          f"dJ/du_in rel {eu:.2e}; phases ms {ph}")
+    # Measured: J 2.6e-11, dJ/dalpha 2.3e-8 (its rounding to float), dJ/du_in 3.6e-12.
+    assert ej <= 1e-9
+    assert eg <= 1e-6
+    assert eu <= 1e-10
+
+
 def test_checkpoint_interval_does_not_change_result():
     """Every checkpoint interval and trace budget -- all segments resident,
     none (every segment recomputed from its checkpoint), or the last two --
@@ -91,10 +116,11 @@ def test_adjoint_leaves_initial_state_and_objective_agrees():
 
 def test_unsupported_configurations_rejected():
     c, flags, rho0, u0, alpha, _ = small_case()
-    with core.Solver(flags, c.nx, c.ny, c.options(storage="fp16s"), rho0=rho0, u0=u0) as s:
-        with pytest.raises(core.FlbError) as e:
-            s.adjoint(10, c.nx - 2, 0, 9)
-        assert e.value.status == 5
+    for storage in ("fp16s", "bf16s"):
+        with core.Solver(flags, c.nx, c.ny, c.options(storage=storage), rho0=rho0, u0=u0) as s:
+            with pytest.raises(core.FlbError) as e:
+                s.adjoint(10, c.nx - 2, 0, 9)
+            assert e.value.status == 5
     with gpu_solver(c, flags, rho0, u0, alpha) as s:
         s.step(3)
         with pytest.raises(core.FlbError):

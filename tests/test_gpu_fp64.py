@@ -133,12 +133,8 @@ def test_float_readback_of_fp64_handle():
     np.testing.assert_array_equal(f32, f64.astype(np.float32))
 
 
-def test_fp64_not_offered_for_adjoint_and_batch():
-    case = cases.scaled(SP, 8)
-    flags, rho0, u0 = case.build()
-    with case.solver("fp64") as s:
-        with pytest.raises(core.FlbError):
-            s.adjoint(10, case.nx - 2, 5, 9, ckpt=5)
+def test_fp64_not_offered_for_batch():
+    # (The adjoint takes FP64 handles: test_gpu_adjoint.py.)
     c = cases.scaled(cases.batch_instances(1, seed=1)[0], 4)
     fl, r0, v0 = c.build()
     with pytest.raises(core.FlbError):
