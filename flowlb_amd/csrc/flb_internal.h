@@ -82,10 +82,14 @@ struct flb_handle {
   unsigned long long* box_host = nullptr;      // host view
   unsigned long long* box_host_dev = nullptr;  // its device address (Params::box_host)
   bool seam_y_closed = false, seam_x_closed = false;  // a wall row / column on the edge
-  // flb_step keeps at most kInFlight steps queued, so that the box the host
-  // sizes a launch from is at most a few steps old.
-  static constexpr int kInFlight = 8;
-  hipEvent_t in_flight[kInFlight] = {};
+  // flb_step keeps at most 2 kGroup steps queued, so that the box the host
+  // sizes a launch from is at most a few steps old: it records an event after
+  // every kGroup-th step (ring of kRing) and, starting a group, waits for the
+  // event two groups back.
+  static constexpr int kGroup = 4;
+  static constexpr int kRing = 4;
+  static constexpr int kInFlight = 2 * kGroup;  // queued steps at most
+  hipEvent_t in_flight[kRing] = {};
   int cus = 256;  // compute units of the device (grid size of the list-driven pass 1)
   float* alpha = nullptr;     // Brinkman solid fraction (allocated on first use)
   float* rho_init = nullptr;  // rho0 and u0 as given to flb_create
