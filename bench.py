@@ -302,8 +302,20 @@ def bench_batch(args):
         flags, rho0, u0 = c.build()
         built.append(dict(nx=c.nx, ny=c.ny, tau=c.tau, flags=flags, rho0=rho0, u0=u0))
     ncells = sum(c.N for c in insts)
+    # Algorithmic bytes of a step: 72 population bytes + 1 flag byte per fluid
+    # cell, 1 flag byte per wall cell.
+    active = sum(int(np.count_nonzero((d["flags"] & 0x0A) == 0)) for d in built)
+    alg_bytes = active * 73 + (ncells - active)
     with core.Batch(built) as b:
-        bx, grid, tune_ms = b.autotune(steps=6)
+        if args.batch_config:
+            # A fixed configuration instead of the autotuner (profiling runs):
+            # order,block_x,grid with grid 0 = one block per tile.
+            order, bx, grid = args.batch_config.split(",")
+            bx, grid = int(bx), int(grid)
+            b.set_order(order)
+            b.configure(block_x=bx, grid=grid)
+        else:
+            bx, grid, tune_ms = b.autotune(steps=6)
         b.step(args.warmup)
         b.sync()
         ms = b.step_timed(args.steps)
@@ -312,7 +324,7 @@ def bench_batch(args):
     # one at a time: every instance on its own handle, per-step launch + sync
     per_sync, per_async = 0.0, 0.0
     k = args.single_steps
-    for c, d in zip(insts, built):
+    for c, d in zip(insts if k > 0 else [], built):
         with core.Solver(d["flags"], c.nx, c.ny, c.options(), rho0=d["rho0"], u0=d["u0"]) as s:
             s.step(2)
             s.sync()
@@ -330,19 +342,30 @@ def bench_batch(args):
            "ms_per_step": round(batched_ms, 5), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
            "config": {"workload": f"batch_{args.instances}", "instances": args.instances,
-                      "cells": ncells, "autotuned_block_x": bx,
+                      "cells": ncells, "autotuned_block_x": bx, "fixed_configuration": bool(args.batch_config),
                       "autotuned_mode": "persistent" if grid < 0 else "launch per step",
                       "autotuned_grid": (f"{-grid} resident blocks per CU" if grid < 0 else
                                          ("one block per tile" if grid == 0 else f"{grid} blocks")),
                       "autotuned_tile_order": info["order"],
                       "steps_per_graph_or_cooperative_launch": info["graph_chunk"],
                       "tiles": info["tiles"]},
-           "one_at_a_time_ms_per_step": round(per_sync * 1e3, 4),
-           "one_at_a_time_async_ms_per_step": round(per_async * 1e3, 4),
-           "speedup_vs_one_at_a_time": round(per_sync * 1e3 / batched_ms, 3),
-           "speedup_vs_one_at_a_time_async": round(per_async * 1e3 / batched_ms, 3),
-           "achieved_GBs": round(ncells * 73 / (batched_ms * 1e-3) / 1e9, 1),
-           "one_at_a_time_async_GBs": round(ncells * 73 / (per_async * 1e3 * 1e-3) / 1e9, 1)}
+           "active_fraction": round(active / ncells, 4),
+           "achieved_GBs": round(alg_bytes / (batched_ms * 1e-3) / 1e9, 1)}
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_batch_{args.instances}_fp32.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    out["roofline"] = {"bound": "hbm", "achieved": out["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(out["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                       "traffic": traffic, "kernel": "batched stream-collide",
+                       "algorithmic_bytes_per_launch": alg_bytes,
+                       "timing": "HIP events around the timed steps (graph replays)"}
+    if k > 0:
+        out.update({"one_at_a_time_ms_per_step": round(per_sync * 1e3, 4),
+                    "one_at_a_time_async_ms_per_step": round(per_async * 1e3, 4),
+                    "speedup_vs_one_at_a_time": round(per_sync * 1e3 / batched_ms, 3),
+                    "speedup_vs_one_at_a_time_async": round(per_async * 1e3 / batched_ms, 3),
+                    "one_at_a_time_async_GBs": round(alg_bytes / (per_async * 1e3 * 1e-3) / 1e9, 1)})
     print(json.dumps(out), flush=True)
 
 
@@ -466,7 +489,10 @@ def main():
                     help="skip the free-surface line under a moving interface")
     ap.add_argument("--fd-cells", type=int, default=16)
     ap.add_argument("--instances", type=int, default=256)
-    ap.add_argument("--single-steps", type=int, default=20)
+    ap.add_argument("--single-steps", type=int, default=20,
+                    help="batch: steps per handle of the one-at-a-time comparison (0: skip it)")
+    ap.add_argument("--batch-config", default="",
+                    help="batch: order,block_x,grid instead of autotuning (e.g. bands,512,0)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the workload is one domain on one MI355X; "
