@@ -106,7 +106,7 @@ __device__ __forceinline__ void block_add(double v, double* target) {
 template <class T, int ODD>
 __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjParams a) {
   uint32_t tx, y;
-  block_tile(blockIdx.x, p.tiles_x, p.ny, tx, y);
+  block_tile(blockIdx.x, p.tiles_x, p.ny, p.band_magic, p.last_magic, tx, y);
   const uint32_t x = tx * blockDim.x + threadIdx.x;
   double duin = 0.0;
   if (x < p.nx) {
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(256) k_adjoint_step(const Params p, const AdjP
 // with exact weights for every storage format.
 __global__ void __launch_bounds__(256) k_adjoint_inlet_init(const Params p, const AdjParams a) {
   uint32_t tx, y;
-  block_tile(blockIdx.x, p.tiles_x, p.ny, tx, y);
+  block_tile(blockIdx.x, p.tiles_x, p.ny, p.band_magic, p.last_magic, tx, y);
   const uint32_t x = tx * blockDim.x + threadIdx.x;
   double acc = 0.0;
   if (x < p.nx) {

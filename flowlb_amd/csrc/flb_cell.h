@@ -88,7 +88,7 @@ struct Cell {
 // Cell of this thread: block = one row segment of blockDim.x cells.
 __device__ __forceinline__ Cell locate(const Params& p) {
   uint32_t tx, y;
-  block_tile(blockIdx.x, p.tiles_x, p.ny, tx, y);
+  block_tile(blockIdx.x, p.tiles_x, p.ny, p.band_magic, p.last_magic, tx, y);
   const uint32_t x = tx * blockDim.x + threadIdx.x;
   return Cell{x, y, x + y * p.pitch, x < p.nx};
 }

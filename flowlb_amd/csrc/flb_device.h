@@ -220,6 +220,10 @@ struct Params {
   // kBandRows rows from band0, block columns col0 .. col0 + ncols - 1.  The
   // whole grid is 0, 0, tiles_x.
   uint32_t band0, col0, ncols;
+  // div_magic(kBandRows * tiles_x), div_magic(kBandRows * ncols) and
+  // div_magic(height of the last band): block_tile and block_tile_window
+  // divide on the scalar unit (0 = divide instead).
+  uint32_t band_magic, win_magic, last_magic;
 };
 
 enum : int { kListFS1 = 0 };
@@ -262,16 +266,49 @@ __device__ __forceinline__ R* fld(float* p) {
 // row-major order.
 constexpr uint32_t kTileOrder = 2;  // 0 row-major, 1 rows rotated by y, 2 bands
 constexpr uint32_t kBandRows = 32;
-__device__ __forceinline__ void block_tile(uint32_t b, uint32_t tiles_x, uint32_t rows,
-                                           uint32_t& tx, uint32_t& y) {
+
+// Division of a wave-uniform index by a launch constant d >= 2 on the scalar
+// unit.  With m = ceil(2^32 / d), q = (n m) >> 32 is floor(n / d) or one more
+// for any n < 2^31 (m exceeds 2^32 / d by less than 1), and the compare
+// corrects it.  The compiler's own unsigned division goes through the vector
+// unit's float reciprocal: ~10 of the ~185 vector instructions of a 16-bit
+// stream-collide wavefront.  m = 0 divides instead.
+__host__ __device__ inline uint32_t div_magic(uint64_t d) {
+  return d >= 2 ? uint32_t(((1ull << 32) + d - 1) / d) : 0u;
+}
+__host__ __device__ __forceinline__ uint32_t div_by(uint32_t n, uint32_t d, uint32_t m) {
+  if (m == 0) return n / d;
+  const uint32_t q = uint32_t((uint64_t(n) * m) >> 32);
+  return q * d > n ? q - 1 : q;
+}
+// Row and block column of block b within a band of h rows: full bands (h =
+// kBandRows) divide by a power of two, the last band (h = last, its magic
+// mlast) by magic multiplication; both are formed and one is selected, so the
+// mapping stays branch-free scalar code.
+__host__ __device__ __forceinline__ void band_cell(uint32_t r, uint32_t h, uint32_t last,
+                                                   uint32_t mlast, uint32_t& c, uint32_t& row) {
+  const uint32_t cl = div_by(r, last, mlast);
+  const bool full = h == kBandRows;
+  c = full ? r / kBandRows : cl;
+  row = full ? r % kBandRows : r - cl * last;
+}
+// Height of the last band of a grid of `rows` rows and its magic.
+__host__ __device__ inline uint32_t last_band_rows(uint32_t rows) {
+  return rows - (rows - 1) / kBandRows * kBandRows;
+}
+
+__host__ __device__ __forceinline__ void block_tile(uint32_t b, uint32_t tiles_x, uint32_t rows,
+                                                    uint32_t magic, uint32_t last_magic,
+                                                    uint32_t& tx, uint32_t& y) {
   if (kTileOrder == 2) {
     const uint32_t per_band = kBandRows * tiles_x;
-    const uint32_t band = b / per_band;
+    const uint32_t band = div_by(b, per_band, magic);
     const uint32_t r = b - band * per_band;
     const uint32_t y0 = band * kBandRows;
     const uint32_t h = rows - y0 < kBandRows ? rows - y0 : kBandRows;
-    tx = r / h;
-    y = y0 + (r - tx * h);
+    uint32_t row;
+    band_cell(r, h, last_band_rows(rows), last_magic, tx, row);
+    y = y0 + row;
     return;
   }
   y = b / tiles_x;
@@ -282,16 +319,17 @@ __device__ __forceinline__ void block_tile(uint32_t b, uint32_t tiles_x, uint32_
 // block_tile (band order) restricted to the window band0 / col0 / ncols of
 // Params: the same mapping as block_tile over the whole grid when the window
 // is (0, 0, tiles_x).
-__device__ __forceinline__ void block_tile_window(const Params& p, uint32_t b, uint32_t& tx,
-                                                  uint32_t& y) {
+__host__ __device__ __forceinline__ void block_tile_window(const Params& p, uint32_t b,
+                                                           uint32_t& tx, uint32_t& y) {
   const uint32_t per_band = kBandRows * p.ncols;
-  const uint32_t band = b / per_band;
+  const uint32_t band = div_by(b, per_band, p.win_magic);
   const uint32_t r = b - band * per_band;
   const uint32_t y0 = (p.band0 + band) * kBandRows;
   const uint32_t h = p.ny - y0 < kBandRows ? p.ny - y0 : kBandRows;
-  const uint32_t c = r / h;
+  uint32_t c, row;
+  band_cell(r, h, last_band_rows(p.ny), p.last_magic, c, row);
   tx = p.col0 + c;
-  y = y0 + (r - c * h);
+  y = y0 + row;
 }
 
 }  // namespace flb

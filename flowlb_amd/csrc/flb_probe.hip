@@ -37,7 +37,7 @@ struct ProbeArgs {
 template <class T, int KIND, int ODD>
 __global__ void __launch_bounds__(kProbeBlock) k_probe(const ProbeArgs a) {
   uint32_t tx, y;
-  flb::block_tile(blockIdx.x, a.p.tiles_x, a.p.ny, tx, y);
+  flb::block_tile(blockIdx.x, a.p.tiles_x, a.p.ny, a.p.band_magic, a.p.last_magic, tx, y);
   const uint32_t x = tx * kProbeBlock + threadIdx.x;
   if (x >= a.p.nx) return;
   const uint64_t Np = a.p.Np;
@@ -96,6 +96,8 @@ extern "C" int flb_bandwidth_probe(int32_t device, uint32_t nx, uint32_t ny, int
   a.p.pitch = (nx + 63u) / 64u * 64u;
   a.p.tiles_x = (nx + kProbeBlock - 1) / kProbeBlock;
   a.p.Np = flb::plane_stride(uint64_t(a.p.pitch) * ny, elem);
+  a.p.band_magic = flb::div_magic(uint64_t(flb::kBandRows) * a.p.tiles_x);
+  a.p.last_magic = flb::div_magic(flb::last_band_rows(ny));
   a.one = 1.0f;
   if ((uint64_t(a.p.pitch) + 128) * ny > 0xffffffffull / 2) return FLB_ERR_ARGUMENT;
   if (hipSetDevice(device) != hipSuccess) return FLB_ERR_DEVICE;
