@@ -1,0 +1,61 @@
+// Host check of the block-to-tile mappings of flb_device.h: block_tile and
+// block_tile_window with the scalar magic division give the tiles of the plain
+// division form for every block index of many grid shapes, and div_by equals
+// n / d.  Built and run by tests/test_block_mapping.py.
+#include <cstdio>
+#include <cstdint>
+#include "flb_device.h"
+using namespace flb;
+static void old_tile(uint32_t b, uint32_t tiles_x, uint32_t rows, uint32_t& tx, uint32_t& y) {
+  const uint32_t per_band = kBandRows * tiles_x;
+  const uint32_t band = b / per_band;
+  const uint32_t r = b - band * per_band;
+  const uint32_t y0 = band * kBandRows;
+  const uint32_t h = rows - y0 < kBandRows ? rows - y0 : kBandRows;
+  tx = r / h;
+  y = y0 + (r - tx * h);
+}
+static void old_window(const Params& p, uint32_t b, uint32_t& tx, uint32_t& y) {
+  const uint32_t per_band = kBandRows * p.ncols;
+  const uint32_t band = b / per_band;
+  const uint32_t r = b - band * per_band;
+  const uint32_t y0 = (p.band0 + band) * kBandRows;
+  const uint32_t h = p.ny - y0 < kBandRows ? p.ny - y0 : kBandRows;
+  const uint32_t c = r / h;
+  tx = p.col0 + c;
+  y = y0 + (r - c * h);
+}
+int main() {
+  uint64_t checked = 0; int bad = 0;
+  const uint32_t txs[] = {1, 2, 3, 4, 5, 7, 8, 16, 31, 32, 33, 64, 100, 128, 255, 256, 1000, 4096};
+  const uint32_t rs[] = {2, 3, 31, 32, 33, 63, 64, 65, 100, 128, 129, 256, 1000, 2048, 2049};
+  for (uint32_t t : txs) for (uint32_t rows : rs) {
+    const uint64_t nb = uint64_t(t) * rows;
+    if (nb > (1u << 24)) continue;
+    const uint32_t m = div_magic(uint64_t(kBandRows) * t);
+    for (uint32_t b = 0; b < nb; ++b) {
+      uint32_t a1, b1, a2, b2;
+      old_tile(b, t, rows, a1, b1);
+      block_tile(b, t, rows, m, div_magic(last_band_rows(rows)), a2, b2);
+      if (a1 != a2 || b1 != b2) { if (bad++ < 5) printf("tile mismatch t=%u rows=%u b=%u\n", t, rows, b); }
+      ++checked;
+      // window: band0 1, col0 2, ncols t (a window inside a wider grid)
+      Params p{}; p.ny = rows + 32; p.band0 = 1; p.col0 = 2; p.ncols = t; p.last_magic = div_magic(last_band_rows(p.ny));
+      p.win_magic = div_magic(uint64_t(kBandRows) * t);
+      uint32_t c1, y1, c2, y2;
+      block_tile_window(p, b, c2, y2);
+      old_window(p, b, c1, y1);
+      if (c1 != c2 || y1 != y2) { if (bad++ < 5) printf("window mismatch t=%u rows=%u b=%u\n", t, rows, b); }
+    }
+  }
+  // div_by against n / d on large numerators
+  for (uint64_t d = 2; d < 200000; d += 37) {
+    const uint32_t m = div_magic(d);
+    for (uint32_t n : {0u, 1u, uint32_t(d - 1), uint32_t(d), uint32_t(d + 1), 0x7fffffffu, 0x7ffffffeu, 123456789u}) {
+      if (div_by(n, uint32_t(d), m) != n / uint32_t(d)) { if (bad++ < 5) printf("div mismatch n=%u d=%llu\n", n, (unsigned long long)d); }
+      ++checked;
+    }
+  }
+  printf("checked %llu, mismatches %d\n", (unsigned long long)checked, bad);
+  return bad != 0;
+}
