@@ -47,10 +47,10 @@ def algorithmic_bytes(active, inactive, s, free_surface):
     stores its density instead of the share and flag: 1 byte less); the
     interface cells' extra velocity store and reconstructed populations
     (< 0.2 % of the cells) are left out; 1 flag byte per inactive cell.  With
-    16-bit storage a free-surface cell also reads and writes its 4-byte
+    16-bit storage a free-surface cell also reads and writes its 2-byte
     rounding residual (the mass the population stores rounded away).
-    Free-surface step: 18 s + 38 per active cell (+ 8 with 16-bit storage)."""
-    resid = 8 if free_surface and s < 4 else 0
+    Free-surface step: 18 s + 38 per active cell (+ 4 with 16-bit storage)."""
+    resid = 4 if free_surface and s < 4 else 0
     sc = active * (18 * s + 1 + (13 if free_surface else 0) + resid) + inactive
     step = active * (18 * s + (38 if free_surface else 1) + resid) + inactive
     return sc, step
