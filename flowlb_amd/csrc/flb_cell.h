@@ -97,8 +97,8 @@ constexpr float kResidScale = 1024.0f;
 __device__ __forceinline__ float resid_load(const Params& p, uint32_t n) {
   return float(at(reinterpret_cast<const _Float16*>(p.rec), n)) * (1.0f / kResidScale);
 }
-__device__ __forceinline__ void resid_store(const Params& p, uint32_t n, double r) {
-  at(reinterpret_cast<_Float16*>(p.rec), n) = _Float16(float(r) * kResidScale);
+__device__ __forceinline__ void resid_store(const Params& p, uint32_t n, float r) {
+  at(reinterpret_cast<_Float16*>(p.rec), n) = _Float16(r * kResidScale);
 }
 
 // Cell of this thread: block = one row segment of blockDim.x cells.
@@ -412,20 +412,35 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
     // Deviation m - rho that link-by-link accounting would give: the pulled
     // shares, plus the rounding of rho itself, plus (16-bit storage) the mass
     // the previous step's stores rounded away.
-    double m = 0.0;
+    R dev;
+    if (kRounded) {
+      // 16-bit storage, in float: the FP64 sums below were a third of the
+      // vector instructions of a body-of-fluid cell, and the 16-bit kernel is
+      // bound by instruction issue.  rho - 1 is exact for rho in [0.5, 2], so
+      // rho1 - (rho - 1) is exactly the rounding of 1 + rho1 (Fast2Sum) -- the
+      // term the FP64 sum is there for.  Left out are the roundings of the float
+      // sums of rho1 and of the shares (~1e-10), far below the binary16
+      // residual's own (~4e-9).
+      R sx = ex[1];
 #pragma unroll
-    for (int i = 1; i < 9; i++) m += double(ex[i]);
-    double sg = 0.0;
+      for (int i = 2; i < 9; i++) sx += ex[i];
+      dev = sx + (rho1 - (rho - R(1))) + resid;
+    } else {
+      double m = 0.0;
 #pragma unroll
-    for (int i = 0; i < 9; i++) sg += double(g[i]);
-    m += (1.0 + sg) - double(rho);
-    if (kRounded) m += double(resid);
+      for (int i = 1; i < 9; i++) m += double(ex[i]);
+      double sg = 0.0;
+#pragma unroll
+      for (int i = 0; i < 9; i++) sg += double(g[i]);
+      m += (1.0 + sg) - double(rho);
+      dev = R(m);
+    }
     if (body) {
       at(fld<R>(p.mass), n) = rho;
-      at(fld<R>(p.excess_out), n) = R(m) / R(8.0f);
+      at(fld<R>(p.excess_out), n) = dev / R(8.0f);
       p.flags_next[n] = fl;
     } else {
-      at(fld<R>(p.mass), n) = R(m);
+      at(fld<R>(p.mass), n) = dev;
     }
   }
   if (iface) {  // interface -> fluid / gas test (:353-367)
@@ -567,11 +582,13 @@ __device__ __forceinline__ void stream_collide_cell(const Params& p, const Cell 
   }
   if (kRounded && cons && !(fl & kInlet)) {
     // Residual of this step's collision and storage rounding, rho - 1 - sum of
-    // the stored populations, for the next step's interior-fluid deviation.
-    double so = 0.0;
+    // the stored populations, for the next step's interior-fluid deviation
+    // (float sum, as the deviation above: its rounding, ~1e-10, is far below
+    // the binary16 store's).
+    R so = S::round(g[0]);
 #pragma unroll
-    for (int i = 0; i < 9; i++) so += double(S::round(g[i]));
-    resid_store(p, n, (double(rho) - 1.0) - so);
+    for (int i = 1; i < 9; i++) so += S::round(g[i]);
+    resid_store(p, n, (rho - R(1)) - so);
   }
   store_outgoing<S, ODD>(p, nb, g);
 }
