@@ -495,8 +495,8 @@ def main():
                     help="batch: order,block_x,grid instead of autotuning (e.g. bands,512,0)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the workload is one domain on one MI355X; "
-              "multi-GPU runs are not part of this build", file=sys.stderr)
+        print("bench.py measures one GPU: the workload is one domain on one MI355X, and "
+              "the solver has no multi-GPU path (--gpus 1 only)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "copy_probe":
