@@ -1,10 +1,12 @@
-// Host check of the block-to-tile mappings of flb_device.h: block_tile and
-// block_tile_window with the scalar magic division give the tiles of the plain
-// division form for every block index of many grid shapes, and div_by equals
-// n / d.  Built and run by tests/test_block_mapping.py.
+// Host checks of the index arithmetic of flb_device.h / flb_internal.h:
+// block_tile and block_tile_window with the scalar magic division give the
+// tiles of the plain division form for every block index of many grid shapes,
+// div_by equals n / d, and plane_stride keeps planes 256-byte aligned, at least
+// a plane long and 4 KiB past a multiple of 8 KiB apart.  Built and run by
+// tests/test_block_mapping.py.
 #include <cstdio>
 #include <cstdint>
-#include "flb_device.h"
+#include "flb_internal.h"
 using namespace flb;
 static void old_tile(uint32_t b, uint32_t tiles_x, uint32_t rows, uint32_t& tx, uint32_t& y) {
   const uint32_t per_band = kBandRows * tiles_x;
@@ -53,6 +55,16 @@ int main() {
     const uint32_t m = div_magic(d);
     for (uint32_t n : {0u, 1u, uint32_t(d - 1), uint32_t(d), uint32_t(d + 1), 0x7fffffffu, 0x7ffffffeu, 123456789u}) {
       if (div_by(n, uint32_t(d), m) != n / uint32_t(d)) { if (bad++ < 5) printf("div mismatch n=%u d=%llu\n", n, (unsigned long long)d); }
+      ++checked;
+    }
+  }
+  for (uint32_t elem : {2u, 4u, 8u}) {
+    for (uint64_t cells : {64ull, 4096ull, 8192ull * 2048ull, 8192ull * 2048ull + 64ull,
+                           1024ull * 256ull, 2112ull * 512ull, 123456ull * 64ull}) {
+      const uint64_t np = plane_stride(cells, elem), bytes = np * elem;
+      if (np < cells || bytes % 256 != 0 || bytes % 8192 != 4096 || bytes - cells * elem >= 12288) {
+        if (bad++ < 5) printf("plane_stride cells=%llu elem=%u -> %llu\n", (unsigned long long)cells, elem, (unsigned long long)np);
+      }
       ++checked;
     }
   }

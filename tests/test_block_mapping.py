@@ -1,7 +1,9 @@
-"""Block-to-tile mapping (flb_device.h): the scalar magic-number division of
-block_tile / block_tile_window gives exactly the tiles of the plain division
-form, for every block index of 270 grid shapes (36 M cases), and div_by equals
-n / d.  Host build with hipcc; no GPU."""
+"""Index arithmetic of the kernels, checked on the host: the scalar
+magic-number division of block_tile / block_tile_window (flb_device.h) gives
+exactly the tiles of the plain division form for every block index of 270 grid
+shapes (36 M cases), div_by equals n / d, and plane_stride (flb_internal.h)
+keeps population planes 256-byte aligned, at least a plane long and 4 KiB past
+a multiple of 8 KiB apart.  Host build with hipcc; no GPU."""
 import subprocess
 from pathlib import Path
 
