@@ -6,8 +6,12 @@
 // tests/test_block_mapping.py.
 #include <cstdio>
 #include <cstdint>
+#include <vector>
 #include "flb_internal.h"
 using namespace flb;
+// The reference form: plain divisions; in a full band of 32 rows block q of a
+// column takes row (q mod 8) * 4 + q / 8 (XCD k = q mod 8 gets rows 4k..4k+3).
+static uint32_t band_row(uint32_t q, uint32_t h) { return h == 32 ? (q % 8) * 4 + q / 8 : q; }
 static void old_tile(uint32_t b, uint32_t tiles_x, uint32_t rows, uint32_t& tx, uint32_t& y) {
   const uint32_t per_band = kBandRows * tiles_x;
   const uint32_t band = b / per_band;
@@ -15,7 +19,7 @@ static void old_tile(uint32_t b, uint32_t tiles_x, uint32_t rows, uint32_t& tx, 
   const uint32_t y0 = band * kBandRows;
   const uint32_t h = rows - y0 < kBandRows ? rows - y0 : kBandRows;
   tx = r / h;
-  y = y0 + (r - tx * h);
+  y = y0 + band_row(r - tx * h, h);
 }
 static void old_window(const Params& p, uint32_t b, uint32_t& tx, uint32_t& y) {
   const uint32_t per_band = kBandRows * p.ncols;
@@ -25,7 +29,7 @@ static void old_window(const Params& p, uint32_t b, uint32_t& tx, uint32_t& y) {
   const uint32_t h = p.ny - y0 < kBandRows ? p.ny - y0 : kBandRows;
   const uint32_t c = r / h;
   tx = p.col0 + c;
-  y = y0 + (r - c * h);
+  y = y0 + band_row(r - c * h, h);
 }
 int main() {
   uint64_t checked = 0; int bad = 0;
@@ -49,6 +53,19 @@ int main() {
       old_window(p, b, c1, y1);
       if (c1 != c2 || y1 != y2) { if (bad++ < 5) printf("window mismatch t=%u rows=%u b=%u\n", t, rows, b); }
     }
+  }
+  // every tile of a grid is visited exactly once
+  for (uint32_t t : {1u, 3u, 32u, 33u}) for (uint32_t rows : {31u, 32u, 64u, 100u}) {
+    const uint32_t m = div_magic(uint64_t(kBandRows) * t), ml = div_magic(last_band_rows(rows));
+    std::vector<int> seen(size_t(t) * rows, 0);
+    for (uint32_t b = 0; b < t * rows; ++b) {
+      uint32_t tx, y;
+      block_tile(b, t, rows, m, ml, tx, y);
+      if (tx >= t || y >= rows) { if (bad++ < 5) printf("out of grid t=%u rows=%u b=%u\n", t, rows, b); continue; }
+      seen[size_t(y) * t + tx]++;
+    }
+    for (int v : seen) if (v != 1) { if (bad++ < 5) printf("tile not visited once t=%u rows=%u\n", t, rows); break; }
+    ++checked;
   }
   // div_by against n / d on large numerators
   for (uint64_t d = 2; d < 200000; d += 37) {
