@@ -238,7 +238,10 @@ class DeviceBuffer:
         if hip().hipMalloc(C.byref(p), self.nbytes) != 0:
             raise FlbError(-1, f"hipMalloc of {self.nbytes} bytes failed")
         self.ptr = p.value
-        if fill is not None and hip().hipMemset(C.c_void_p(self.ptr), int(fill), self.nbytes):
+        # The solver's stream does not wait for the null stream, so the fill
+        # (and from_host's copy) completes before a solver call can use the buffer.
+        if fill is not None and (hip().hipMemset(C.c_void_p(self.ptr), int(fill), self.nbytes)
+                                 or hip().hipDeviceSynchronize()):
             raise FlbError(-1, "hipMemset failed")
 
     def to_host(self, dtype=np.uint8):
@@ -252,7 +255,8 @@ class DeviceBuffer:
         a = np.ascontiguousarray(a)
         if a.nbytes > self.nbytes:
             raise ValueError("array larger than the buffer")
-        if hip().hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes, self.H2D):
+        if (hip().hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes, self.H2D)
+                or hip().hipDeviceSynchronize()):
             raise FlbError(-1, "hipMemcpy to device failed")
 
     def close(self):
