@@ -199,10 +199,21 @@ int build_tiles(flb_batch* b, int bx) {
         for (uint32_t r = 0; r < p.ny; ++r)
           for (uint32_t k = 0; k < tx; ++k) tl.push_back(flb::Tile{i, r, k, 0});
       } else {
-        for (uint32_t r0 = 0; r0 < p.ny; r0 += kBand)
-          for (uint32_t k = 0; k < tx; ++k)
-            for (uint32_t r = r0; r < std::min(p.ny, r0 + kBand); ++r)
+        // In a full band the tile at list position q runs on XCD q mod 8 (one
+        // block per tile, or a grid-stride walk over a multiple of 8 blocks) and
+        // takes row (q mod 8) * 4 + j / 8 of its column (j its place in the
+        // column): each XCD gets four adjacent rows, as in the single-domain
+        // band order, so that row-neighbour reads stay in one L2.
+        for (uint32_t r0 = 0; r0 < p.ny; r0 += kBand) {
+          const uint32_t h = std::min(p.ny - r0, kBand);
+          for (uint32_t k = 0; k < tx; ++k) {
+            const uint32_t base = uint32_t(tl.size());
+            for (uint32_t j = 0; j < h; ++j) {
+              const uint32_t r = h == kBand ? r0 + ((base + j) % 8u) * 4u + j / 8u : r0 + j;
               tl.push_back(flb::Tile{i, r, k, 0});
+            }
+          }
+        }
       }
     }
   }
