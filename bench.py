@@ -302,11 +302,12 @@ def bench_batch(args):
         flags, rho0, u0 = c.build()
         built.append(dict(nx=c.nx, ny=c.ny, tau=c.tau, flags=flags, rho0=rho0, u0=u0))
     ncells = sum(c.N for c in insts)
-    # Algorithmic bytes of a step: 72 population bytes + 1 flag byte per fluid
+    # Algorithmic bytes of a step: 18 population values + 1 flag byte per fluid
     # cell, 1 flag byte per wall cell.
     active = sum(int(np.count_nonzero((d["flags"] & 0x0A) == 0)) for d in built)
-    alg_bytes = active * 73 + (ncells - active)
-    with core.Batch(built) as b:
+    per_cell = 18 * STORAGE_BYTES[args.storage] + 1
+    alg_bytes = active * per_cell + (ncells - active)
+    with core.Batch(built, storage=args.storage) as b:
         if args.batch_config:
             # A fixed configuration instead of the autotuner (profiling runs):
             # order,block_x,grid with grid 0 = one block per tile.
@@ -325,7 +326,8 @@ def bench_batch(args):
     per_sync, per_async = 0.0, 0.0
     k = args.single_steps
     for c, d in zip(insts if k > 0 else [], built):
-        with core.Solver(d["flags"], c.nx, c.ny, c.options(), rho0=d["rho0"], u0=d["u0"]) as s:
+        with core.Solver(d["flags"], c.nx, c.ny, c.options(args.storage), rho0=d["rho0"],
+                         u0=d["u0"]) as s:
             s.step(2)
             s.sync()
             t = time.perf_counter()
@@ -340,8 +342,11 @@ def bench_batch(args):
     out = {"metric": METRIC, "value": round(ncells / (batched_ms * 1e-3) / 1e6, 1), "unit": "MLUPS",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(batched_ms, 5), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "vs_baseline": None,
+           "dtype": "fp32" if args.storage == "fp32" else f"fp32 arithmetic, {args.storage} storage",
+           "data": "synthetic",
            "config": {"workload": f"batch_{args.instances}", "instances": args.instances,
+                      "storage": args.storage,
                       "cells": ncells, "autotuned_block_x": bx, "fixed_configuration": bool(args.batch_config),
                       "autotuned_mode": "persistent" if grid < 0 else "launch per step",
                       "autotuned_grid": (f"{-grid} resident blocks per CU" if grid < 0 else
@@ -352,7 +357,7 @@ def bench_batch(args):
            "active_fraction": round(active / ncells, 4),
            "achieved_GBs": round(alg_bytes / (batched_ms * 1e-3) / 1e9, 1)}
     traffic = None
-    tf = ROOT / "profiles" / f"traffic_batch_{args.instances}_fp32.json"
+    tf = ROOT / "profiles" / f"traffic_batch_{args.instances}_{args.storage}.json"
     if tf.exists():
         traffic = json.loads(tf.read_text()).get("bytes_per_launch")
     out["roofline"] = {"bound": "hbm", "achieved": out["achieved_GBs"], "peak": HBM_PEAK_GBS,
