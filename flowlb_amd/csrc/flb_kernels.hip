@@ -79,29 +79,37 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
   R* const phi = fld<R>(p.phi);
   const uint32_t n = c.n;
   const uint64_t N = p.N;
-  // The own and the eight neighbour flags are loaded together (one memory
-  // round trip); gas, wall and inlet cells then leave.
+  // Every value a fluid or interface cell may need (flags, the 17 population
+  // slots, the neighbours' excess shares and fill levels, the own mass) is
+  // loaded together with the own flag, so that a cell waits for memory once;
+  // gas, wall and inlet cells then leave, and their loads (all in bounds) go
+  // unused.
   uint32_t nb[9];
   neighbours(p, c.x, c.y, nb);
   const uint8_t fl = p.flags[n];
   uint8_t nf[9];
   neighbour_flags(p, nb, nf);
-  if (fl & (kGas | kWall | kInlet)) return;
-  const bool cons = p.mass_mode != 0;
-  // Conservative mode: the mass of a fluid cell whose eight neighbours are all
-  // fluid or interface is updated by stream-collide instead (interior_fluid).
-  if (cons && interior_fluid(fl, nf)) return;
-  R in[9], out[9], ex[9];
+  R in[9], out[9], ex[9], ph[9];
   load_incoming<S, ODD>(p, nb, in);
   out[0] = in[0];
   load_outgoing<S, ODD>(p, nb, out);
 #pragma unroll
   for (int i = 1; i < 9; i++) ex[i] = fld<R>(p.excess)[nb[i]];
+#pragma unroll
+  for (int i = 1; i < 9; i++) ph[i] = phi[nb[i]];
+  const R phi_n = phi[n];
+  R m = mass[n];
+  const bool cons = p.mass_mode != 0;
+  // Conservative mode: the mass of a fluid cell whose eight neighbours are all
+  // fluid or interface is updated by stream-collide instead (interior_fluid).
+  // (One test after all loads: a separate test of the own flag first would let
+  // the compiler move every load behind it, one more wait for memory.)
+  const bool idle = ((fl & (kGas | kWall | kInlet)) != 0) | (cons & interior_fluid(fl, nf));
+  if (idle) return;
   // As written the increments are added to the mass one by one in FP32
   // (lbmKernel2D.cu:38-52, 94-106).  The conservative mode sums them in FP64
   // and rounds once, so excess shares far below the rounding step of the
   // mass are not lost (FP32 one-by-one drifts ~1e-5 relative per 10^4 steps).
-  R m = mass[n];
   double dm = 0.0;
 #pragma unroll
   for (int i = 1; i < 9; i++) {
@@ -157,10 +165,7 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
       ux = over_rho(mx);
       uy = over_rho(my);
     }
-    R ph[9];
-#pragma unroll
-    for (int i = 1; i < 9; i++) ph[i] = phi[nb[i]];
-    const R phi0 = cons ? phi[n] : fill_level<R>(m, rho);
+    const R phi0 = cons ? phi_n : fill_level<R>(m, rho);
     const R rhol2 = over_rho(R(0.5f));
     ux = ux + rhol2 * R(p.fx);
     uy = uy + rhol2 * R(p.fy);
