@@ -303,14 +303,23 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
     uint32_t nb[9];
     neighbours(p, uint32_t(x), uint32_t(y), nb);
     const uint64_t N = p.N;
+    // The 24 loads are issued before any neighbour's flag is tested (a load
+    // under the test would be waited for before the next neighbour's).
+    float nr[9], nu[9], nv[9];
+#pragma unroll
+    for (int k = 1; k < 9; k++) {
+      nr[k] = float(fld<R>(p.rho)[nb[k]]);
+      nu[k] = float(fld<R>(p.u)[nb[k]]);
+      nv[k] = float(fld<R>(p.u)[N + nb[k]]);
+    }
     float cnt = 0.0f, sr = 0.0f, sx = 0.0f, sy = 0.0f;
 #pragma unroll
     for (int k = 1; k < 9; k++) {
       if (f2s[j + lds_offset(k)] & kIF) {
         cnt += 1.0f;
-        sr += float(fld<R>(p.rho)[nb[k]]);
-        sx += float(fld<R>(p.u)[nb[k]]);
-        sy += float(fld<R>(p.u)[N + nb[k]]);
+        sr += nr[k];
+        sx += nu[k];
+        sy += nv[k];
       }
     }
     const R ar = R(cnt > 0.0f ? sr / cnt : 0.0f);
@@ -466,6 +475,10 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
   const uint8_t h1 =
       p.flags[wrap<SMALL>(x0 + c1 - kHalo, nx) + wrap<SMALL>(y0 + r1 - kHalo, ny) * pitch];
   const R rho_pre = fld<R>(p.rho)[n];
+  // Keeps the loads together: the scheduler would otherwise place the test of
+  // the mass between them and wait for it before issuing the halo loads (the
+  // empty statement takes the mass as input, after every load of the tile).
+  asm volatile("" : "+v"(m) : : "memory");
   f0s[i0] = h0;
   f0s[i1] = h1;
   // A flag changes in passes 2-4 only within three cells of a fluid or
@@ -622,6 +635,9 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   uint32_t i = blockIdx.x;
   uint32_t tile = first_entry(p.tile_list, i, p.work_x * p.tile_rows);
   const uint32_t count = *p.tile_count;
+  // Keeps the first entry's load beside the length's: the compiler would
+  // otherwise move it behind the length test, a second wait for memory.
+  asm volatile("" ::: "memory");
   while (i < count) {
     const uint32_t j = i + gridDim.x;
     const uint32_t next = j < count ? p.tile_list[j] : 0u;
