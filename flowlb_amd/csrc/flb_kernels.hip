@@ -270,8 +270,26 @@ __device__ __forceinline__ int lds_offset(int i) {
 // Pass 3's gas->interface initialisation and pass 4 for one cell of the tile:
 // f0 its flag after stream-collide, f2s[j] its flag after pass 3 (neighbours
 // at lds_offset), m its mass.
+// Receiver bits of a cell for its neighbours' excess shares, from its flag
+// after passes 2 and 3 (computed once per cell of the tile and its ring in
+// pass 3; each cell of the tile then adds up its eight neighbours' bits):
+// kRcvWritten -- a receiver as written (fluid, interface, interface->fluid or
+// gas->interface before the transitions, lbmKernel2D.cu:285-292);
+// kRcvCons -- a receiver in the conservative mode (not gas, wall or inlet
+// after its own transition); kRcvFI -- fluid or interface after it.
+constexpr uint8_t kRcvCons = 1, kRcvFI = 2, kRcvWritten = 4;
+__device__ __forceinline__ uint8_t receiver_bits(uint8_t q) {
+  const uint8_t r = q & (kIF | kGasInterface);
+  const bool written = r == kFluid || r == kInterface || r == kIF || r == kGasInterface;
+  const uint8_t sq = settle(q);
+  const bool cons = (sq & (kGas | kWall | kInlet)) == 0;
+  const bool fi = (sq & kFGI) == kFluid || (sq & kFGI) == kInterface;
+  return uint8_t((written ? kRcvWritten : 0) | (cons ? kRcvCons : 0) | (fi ? kRcvFI : 0));
+}
+
 template <class S, int ODD>
-__device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f2s, int j,
+__device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f2s,
+                                               const uint8_t* rcv, int j,
                                                uint8_t f0, typename S::R m, typename S::R rho_pre,
                                                uint32_t n, int x, int y, bool& interior) {
   using R = typename S::R;
@@ -377,26 +395,21 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
     default:
       break;
   }
-  // Receivers of the excess.  As written: neighbours fluid, interface,
-  // interface->fluid or gas->interface before the transitions
+  // Receivers of the excess (receiver_bits).  As written: neighbours fluid,
+  // interface, interface->fluid or gas->interface before the transitions
   // (lbmKernel2D.cu:285-292).  Conservative: the cells that pull it in the
   // next pass 1, i.e. not gas, wall or inlet after their own transition.
   // The same settled flags tell whether all eight neighbours are fluid or
   // interface next step (interior_fluid of the next pass 1).
   int cont = 0;
-  bool all_fi = true;
+  uint8_t all = 0xff;
 #pragma unroll
   for (int k = 1; k < 9; k++) {
-    const uint8_t q = f2s[j + lds_offset(k)];
-    if (!cons) {
-      const uint8_t r = q & (kIF | kGasInterface);
-      cont += int(r == kFluid || r == kInterface || r == kIF || r == kGasInterface);
-    } else {
-      const uint8_t sq = settle(q);
-      cont += int((sq & (kGas | kWall | kInlet)) == 0);
-      all_fi = all_fi && ((sq & kFGI) == kFluid || (sq & kFGI) == kInterface);
-    }
+    const uint8_t q = rcv[j + lds_offset(k)];
+    cont += (q >> (cons ? 0 : 2)) & 1;
+    all &= q;
   }
+  const bool all_fi = (all & kRcvFI) != 0;
   interior = cons && all_fi;
   if (cont > 0) ex = ex / R(float(cont));
   else {
@@ -444,7 +457,7 @@ __device__ __forceinline__ void finish_tile(const Params& p, int tid, uint32_t t
 // Passes 2-4 of one tile (one kFsTileW x kFsTileH block).
 template <class S, int ODD, bool SMALL>
 __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32_t ty,
-                                            uint8_t* f0s, uint8_t* f1s, uint8_t* f2s,
+                                            uint8_t* f0s, uint8_t* f1s, uint8_t* f2s, uint8_t* rcv,
                                             uint32_t* work_bits, uint32_t* busy_waves) {
   using R = typename S::R;
   R* const mass = fld<R>(p.mass);
@@ -551,13 +564,14 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
       if (near_ig) o = uint8_t((fl & ~kIF) | kInterface);
     }
     f2s[j] = o;
+    rcv[j] = receiver_bits(o);
   }
   __syncthreads();
   bool sc = false, fs1 = false, nonfluid = false;
   if (valid) {
     const int j = (tid / kTileW + kHalo) * kLdsW + (tid & (kTileW - 1)) + kHalo;
     bool interior;
-    const uint8_t fin = update_cell<S, ODD>(p, f2s, j, f0s[j], m, rho_pre, n, x, y, interior);
+    const uint8_t fin = update_cell<S, ODD>(p, f2s, rcv, j, f0s[j], m, rho_pre, n, x, y, interior);
     sc = !(fin & (kGas | kWall));
     fs1 = sc && !(interior && (fin & (kFGI | kInlet | kOutlet)) == kFluid);
     nonfluid = fin != kFluid;
@@ -623,6 +637,7 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
   // f0s is padded to two bytes per thread so that every lane writes both of
   // its halo bytes unconditionally (lanes past the tile write the padding).
   __shared__ uint8_t f0s[2 * kUpdateThreads], f1s[kLdsH * kLdsW], f2s[kLdsH * kLdsW];
+  __shared__ uint8_t rcv[kLdsH * kLdsW];  // receiver_bits of f2s
   __shared__ uint32_t work_bits;
   __shared__ uint32_t busy_waves[2];  // one byte per wavefront
   static_assert(kUpdateThreads / 64 <= 8, "one byte per wavefront");
@@ -642,7 +657,7 @@ __global__ void __launch_bounds__(kUpdateThreads) k_fs_update(const Params p) {
     const uint32_t j = i + gridDim.x;
     const uint32_t next = j < count ? p.tile_list[j] : 0u;
     const uint32_t ty = tile / p.work_x, tx = tile - ty * p.work_x;
-    update_tile<S, ODD, SMALL>(p, tx, ty, f0s, f1s, f2s, &work_bits, busy_waves);
+    update_tile<S, ODD, SMALL>(p, tx, ty, f0s, f1s, f2s, rcv, &work_bits, busy_waves);
     i = j;
     tile = next;
   }
