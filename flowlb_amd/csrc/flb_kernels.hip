@@ -94,11 +94,11 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
   out[0] = in[0];
   load_outgoing<S, ODD>(p, nb, out);
 #pragma unroll
-  for (int i = 1; i < 9; i++) ex[i] = fld<R>(p.excess)[nb[i]];
+  for (int i = 1; i < 9; i++) ex[i] = at(fld<R>(p.excess), nb[i]);
 #pragma unroll
-  for (int i = 1; i < 9; i++) ph[i] = phi[nb[i]];
-  const R phi_n = phi[n];
-  R m = mass[n];
+  for (int i = 1; i < 9; i++) ph[i] = at(phi, nb[i]);
+  const R phi_n = at(phi, n);
+  R m = at(mass, n);
   const bool cons = p.mass_mode != 0;
   // Conservative mode: the mass of a fluid cell whose eight neighbours are all
   // fluid or interface is updated by stream-collide instead (interior_fluid).
@@ -135,23 +135,23 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
       // incoming populations in the same order (an outlet keeps its density).
       R rho;
       if (fl & kOutlet) {
-        rho = fld<R>(p.rho)[n];
+        rho = at(fld<R>(p.rho), n);
       } else {
         R s = in[0];
 #pragma unroll
         for (int i = 1; i < 9; i++) s += in[i];
         rho = R(1) + s;
       }
-      mass[n] = R(dm + (double(m) - double(rho)));
+      at(mass, n) = R(dm + (double(m) - double(rho)));
       return;
     }
   } else if (e == kInterface) {
     R rho, ux, uy, mx = R(0), my = R(0);
     const bool io = fl & (kInlet | kOutlet);
     if (io) {
-      rho = fld<R>(p.rho)[n];
-      ux = fld<R>(p.u)[n];
-      uy = fld<R>(p.u)[N + n];
+      rho = at(fld<R>(p.rho), n);
+      ux = at(fld<R>(p.u), n);
+      uy = at(fld<R>(p.u) + N, n);
     } else {
       R s = out[0];
 #pragma unroll
@@ -195,11 +195,11 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
     for (int k = 1; k < 9; k++) {
       const int opp = (k & 1) ? k + 1 : k - 1;
       const uint8_t gate = p.gas_check_corrected ? nf[opp] : nf[k];
-      if (gate == kGas) fld<R>(p.rec)[k * N + n] = fe[k] - out[opp] + fe[opp];
+      if (gate == kGas) at(fld<R>(p.rec) + k * N, n) = fe[k] - out[opp] + fe[opp];
     }
   }
   if (cons) m = R(double(m) + dm);
-  mass[n] = m;
+  at(mass, n) = m;
 }
 
 
@@ -208,7 +208,7 @@ __device__ __forceinline__ void fs1_cell(const Params& p, const Cell c) {
 // the length; the value is used only if i < length): the walker then waits for
 // memory once before its first item instead of twice.
 __device__ __forceinline__ uint32_t first_entry(const uint32_t* list, uint32_t i, uint32_t cap) {
-  return list[i < cap ? i : cap - 1u];
+  return at(list, i < cap ? i : cap - 1u);
 }
 
 // Flag transition a cell applies to itself in pass 4.
@@ -307,9 +307,9 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
     // excess share for its neighbours and marked its mass -0.0; clear both.
     p.flags_next[n] = fl;
     if (signbit(m)) {  // both share buffers: this step's readers are done
-      mass[n] = R(0);
-      fld<R>(p.excess)[n] = R(0);
-      fld<R>(p.excess_out)[n] = R(0);
+      at(mass, n) = R(0);
+      at(fld<R>(p.excess), n) = R(0);
+      at(fld<R>(p.excess_out), n) = R(0);
     }
     return fl;
   }
@@ -326,9 +326,9 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
     float nr[9], nu[9], nv[9];
 #pragma unroll
     for (int k = 1; k < 9; k++) {
-      nr[k] = float(fld<R>(p.rho)[nb[k]]);
-      nu[k] = float(fld<R>(p.u)[nb[k]]);
-      nv[k] = float(fld<R>(p.u)[N + nb[k]]);
+      nr[k] = float(at(fld<R>(p.rho), nb[k]));
+      nu[k] = float(at(fld<R>(p.u), nb[k]));
+      nv[k] = float(at(fld<R>(p.u) + N, nb[k]));
     }
     float cnt = 0.0f, sr = 0.0f, sx = 0.0f, sy = 0.0f;
 #pragma unroll
@@ -416,9 +416,9 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
     m += ex;
     ex = R(0);
   }
-  mass[n] = m;
-  fld<R>(p.excess_out)[n] = ex;
-  if (write_phi) fld<R>(p.phi)[n] = ph;
+  at(mass, n) = m;
+  at(fld<R>(p.excess_out), n) = ex;
+  if (write_phi) at(fld<R>(p.phi), n) = ph;
   p.flags_next[n] = nfl;
   return nfl;
 }
@@ -443,13 +443,13 @@ __device__ __forceinline__ void finish_tile(const Params& p, int tid, uint32_t t
   __syncthreads();
   if (tid == 0) {
     const uint32_t w = (*bits ^ kWorkFluid) | quiet;  // the fluid bit: no wavefront saw another cell
-    if (w != old) p.tile_work[ty * p.work_x + tx] = w;
+    if (w != old) at(p.tile_work, ty * p.work_x + tx) = w;
     const uint32_t rows_fs1 = (w >> kWorkFS1) & 0xffu;
     if (rows_fs1) {
       const uint32_t chunk0 = ty * kFsTileH * p.work_x + tx;
       uint32_t k = atomicAdd(&p.counts_out[kListFS1], uint32_t(__popc(rows_fs1)));
       for (uint32_t m = rows_fs1; m; m &= m - 1)
-        p.fs1_list_out[k++] = chunk0 + uint32_t(__ffs(m) - 1) * p.work_x;
+        at(p.fs1_list_out, k++) = chunk0 + uint32_t(__ffs(m) - 1) * p.work_x;
     }
   }
 }
@@ -467,7 +467,7 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
   const int x = x0 + (tid & (kTileW - 1)), y = y0 + tid / kTileW;
   const bool valid = x < nx && y < ny;
   const uint32_t n = valid ? uint32_t(x + y * pitch) : 0u;
-  const uint32_t old_work = p.tile_work[ty * p.work_x + tx];
+  const uint32_t old_work = at(p.tile_work, ty * p.work_x + tx);
   if (tid == 0) *work_bits = 0;  // ordered before its use by the next barrier
   // All global loads of the tile are issued before any is used: the two halo
   // bytes of this thread, its own flag, mass and density (read for every
@@ -482,12 +482,12 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
   const int r0 = i0 / kLdsW, c0 = i0 - r0 * kLdsW;
   const int r1 = i1c / kLdsW, c1 = i1c - r1 * kLdsW;
   const uint8_t own = p.flags[n];
-  R m = mass[n];
+  R m = at(mass, n);
   const uint8_t h0 =
       p.flags[wrap<SMALL>(x0 + c0 - kHalo, nx) + wrap<SMALL>(y0 + r0 - kHalo, ny) * pitch];
   const uint8_t h1 =
       p.flags[wrap<SMALL>(x0 + c1 - kHalo, nx) + wrap<SMALL>(y0 + r1 - kHalo, ny) * pitch];
-  const R rho_pre = fld<R>(p.rho)[n];
+  const R rho_pre = at(fld<R>(p.rho), n);
   // Keeps the loads together: the scheduler would otherwise place the test of
   // the mass between them and wait for it before issuing the halo loads (the
   // empty statement takes the mass as input, after every load of the tile).
@@ -516,8 +516,8 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
     const bool cons = p.mass_mode != 0;
     if (valid) {
       const R ex = cons ? m : m - rho_pre;
-      mass[n] = rho_pre;
-      fld<R>(p.excess_out)[n] = ex / R(8.0f);
+      at(mass, n) = rho_pre;
+      at(fld<R>(p.excess_out), n) = ex / R(8.0f);
       p.flags_next[n] = own;
     }
     finish_tile(p, tid, tx, ty, old_work, valid, valid && !cons, false, work_bits, 0u);
@@ -528,9 +528,9 @@ __device__ __forceinline__ void update_tile(const Params& p, uint32_t tx, uint32
       p.flags_next[n] = own;
       if (!(own & (kWall | kMovingWall | kInlet)) && (own & (kFGI | kGasInterface)) == kGas &&
           signbit(m)) {  // both share buffers: this step's readers are done
-        mass[n] = R(0);
-        fld<R>(p.excess)[n] = R(0);
-        fld<R>(p.excess_out)[n] = R(0);
+        at(mass, n) = R(0);
+        at(fld<R>(p.excess), n) = R(0);
+        at(fld<R>(p.excess_out), n) = R(0);
       }
     }
     const bool sc = valid && !(own & (kGas | kWall));
@@ -655,7 +655,7 @@ __device__ __forceinline__ void fs_update(const Params& p) {
   asm volatile("" ::: "memory");
   while (i < count) {
     const uint32_t j = i + gridDim.x;
-    const uint32_t next = j < count ? p.tile_list[j] : 0u;
+    const uint32_t next = j < count ? at(p.tile_list, j) : 0u;
     const uint32_t ty = tile / p.work_x, tx = tile - ty * p.work_x;
     update_tile<S, ODD, SMALL>(p, tx, ty, f0s, f1s, f2s, rcv, &work_bits, busy_waves);
     i = j;
@@ -706,7 +706,7 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
   const uint32_t count = chk ? p.counts_in[kListFS1] : 0u;
   for (; i < count; i += nw) {
     const uint32_t j = i + nw;
-    const uint32_t next = j < count ? p.fs1_list[j] : 0u;
+    const uint32_t next = j < count ? at(p.fs1_list, j) : 0u;
     const uint32_t y = c / p.work_x, x = (c - y * p.work_x) * kFsTileW + lane;
     const Cell cell{x, y, x + y * p.pitch, x < p.nx};
     if (cell.valid) fs1_cell<S, ODD>(p, cell);
@@ -735,14 +735,14 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
       const uint32_t xm = tx > 0 ? tx - 1 : p.work_x - 1, xp = tx + 1 < p.work_x ? tx + 1 : 0;
       const uint32_t ym = ty > 0 ? ty - 1 : p.tile_rows - 1;
       const uint32_t yp = ty + 1 < p.tile_rows ? ty + 1 : 0;
-      const uint32_t pub = p.tile_pub[t];
+      const uint32_t pub = at(p.tile_pub, t);
       uint32_t w[9];
       const uint32_t rows[3] = {ym * p.work_x, ty * p.work_x, yp * p.work_x};
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        w[3 * k] = p.tile_work[rows[k] + xm];
-        w[3 * k + 1] = p.tile_work[rows[k] + tx];
-        w[3 * k + 2] = p.tile_work[rows[k] + xp];
+        w[3 * k] = at(p.tile_work, rows[k] + xm);
+        w[3 * k + 1] = at(p.tile_work, rows[k] + tx);
+        w[3 * k + 2] = at(p.tile_work, rows[k] + xp);
       }
       const bool spill = (thin_x && (tx == 0 || tx + 2 == p.work_x)) ||
                          (thin_y && (ty == 0 || ty + 2 == p.tile_rows));
@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
         // and sets their mass, as the update's body-of-fluid path would.
         body = p.mass_mode != 0 && (all & kWorkFluid) && !(own & (0xffu << kWorkFS1));
       }
-      p.tile_body[t] = body;
+      at(p.tile_body, t) = body;
       need = !skip && !body;
     }
     // Bounding box of this wavefront's tiles with stream-collide work (empty:
@@ -790,7 +790,7 @@ __global__ void __launch_bounds__(kListThreads) k_fs1(const Params p) {
     uint32_t base = 0;
     if (lane == 0 && m) base = atomicAdd(p.tile_count, uint32_t(__popcll(m)));
     base = __shfl(base, 0, 64);
-    if (need) p.tile_list[base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)))] = t;
+    if (need) at(p.tile_list, base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)))) = t;
   }
 }
 
