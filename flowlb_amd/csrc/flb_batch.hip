@@ -37,14 +37,24 @@ struct Tile {
 template <class S, int ODD, bool MOM>
 __global__ void __launch_bounds__(1024) k_batch_step(const Tile* __restrict__ tiles, uint32_t ntiles,
                                                       const Params* __restrict__ inst) {
-  for (uint32_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
-    const Tile t = tiles[k];
+  uint32_t k = blockIdx.x;
+  if (k >= ntiles) return;
+  Tile t = tiles[k];
+  // A grid-stride block loads its next tile's descriptor while it evaluates
+  // this one (with a block per tile there is no next: the same line again).
+  for (;;) {
+    const uint32_t kn = k + gridDim.x;
+    const bool more = kn < ntiles;
+    const Tile tn = tiles[more ? kn : k];
     const Params& p = inst[t.inst];
     const uint32_t x = t.tx * blockDim.x + threadIdx.x;
     if (x < p.nx) {
       const Cell c{x, t.row, x + t.row * p.pitch, true};
       stream_collide_cell<S, ODD, false, MOM, false>(p, c);
     }
+    if (!more) break;
+    k = kn;
+    t = tn;
   }
 }
 
