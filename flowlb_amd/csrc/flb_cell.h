@@ -227,11 +227,11 @@ __device__ __forceinline__ typename S::R pop_load(const Params& p, int q, uint32
   if (S::kBuffer) return S::bload(pop_plane<S>(p, q), cell);
   return S::load(static_cast<const T*>(p.f) + uint64_t(q) * p.Np, cell);
 }
-template <class S>
+template <class S, bool BUF = S::kBuffer>
 __device__ __forceinline__ void pop_store(const Params& p, int q, uint32_t cell,
                                           typename S::R v) {
   using T = typename S::T;
-  if (S::kBuffer) S::bstore(pop_plane<S>(p, q), cell, v);
+  if (BUF) S::bstore(pop_plane<S>(p, q), cell, v);
   else S::store(static_cast<T*>(p.f) + uint64_t(q) * p.Np, cell, v);
 }
 
@@ -258,15 +258,17 @@ __device__ __forceinline__ void load_outgoing(const Params& p, const uint32_t* n
   }
 }
 
-// storef (src/cuda/cellCalculations.cuh:228-238).
-template <class S, int ODD>
+// storef (src/cuda/cellCalculations.cuh:228-238).  BUF false: plain stores
+// from each plane's base (two scalar registers a plane instead of a
+// four-register buffer descriptor), for kernels short of scalar registers.
+template <class S, int ODD, bool BUF = S::kBuffer>
 __device__ __forceinline__ void store_outgoing(const Params& p, const uint32_t* nb,
                                                const typename S::R* g) {
-  pop_store<S>(p, 0, nb[0], g[0]);
+  pop_store<S, BUF>(p, 0, nb[0], g[0]);
 #pragma unroll
   for (int i = 1; i < 9; i += 2) {
-    pop_store<S>(p, ODD ? i + 1 : i, nb[i], g[i]);
-    pop_store<S>(p, ODD ? i : i + 1, nb[0], g[i + 1]);
+    pop_store<S, BUF>(p, ODD ? i + 1 : i, nb[i], g[i]);
+    pop_store<S, BUF>(p, ODD ? i : i + 1, nb[0], g[i + 1]);
   }
 }
 

@@ -348,7 +348,7 @@ __device__ __forceinline__ uint8_t update_cell(const Params& p, const uint8_t* f
     uy = uy + rhol2 * R(p.fy);
     R fe[9];
     equilibrium_shifted<R>(ar, ar - R(1), ux, uy, fe);
-    store_outgoing<S, ODD>(p, nb, fe);
+    store_outgoing<S, ODD, false>(p, nb, fe);  // the update is short of scalar registers
   }
   // A fluid cell carries its mass as the deviation from rho in the
   // conservative mode (interior_fluid); one that pass 3 made interface takes
