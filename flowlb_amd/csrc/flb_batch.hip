@@ -37,15 +37,18 @@ struct Tile {
 template <class S, int ODD, bool MOM>
 __global__ void __launch_bounds__(1024) k_batch_step(const Tile* __restrict__ tiles, uint32_t ntiles,
                                                       const Params* __restrict__ inst) {
+  // Odd steps walk the list backwards with 4-byte storage (walk_block).
+  const bool rev = ODD && S::kBytes >= 4;
+  const auto at_pos = [&](uint32_t k) { return tiles[rev ? ntiles - 1u - k : k]; };
   uint32_t k = blockIdx.x;
   if (k >= ntiles) return;
-  Tile t = tiles[k];
+  Tile t = at_pos(k);
   // A grid-stride block loads its next tile's descriptor while it evaluates
   // this one (with a block per tile there is no next: the same line again).
   for (;;) {
     const uint32_t kn = k + gridDim.x;
     const bool more = kn < ntiles;
-    const Tile tn = tiles[more ? kn : k];
+    const Tile tn = at_pos(more ? kn : k);
     const Params& p = inst[t.inst];
     const uint32_t x = t.tx * blockDim.x + threadIdx.x;
     if (x < p.nx) {
@@ -70,8 +73,9 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(const Tile* __restric
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   for (uint32_t s = 0; s < nsteps; ++s) {
     const bool odd = ((odd0 + s) & 1u) != 0;
+    const bool rev = odd && S::kBytes >= 4;  // walk_block's order
     for (uint32_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
-      const Tile t = tiles[k];
+      const Tile t = tiles[rev ? ntiles - 1u - k : k];
       const Params& p = inst[t.inst];
       const uint32_t x = t.tx * blockDim.x + threadIdx.x;
       if (x < p.nx) {

@@ -101,19 +101,31 @@ __device__ __forceinline__ void resid_store(const Params& p, uint32_t n, float r
   at(reinterpret_cast<_Float16*>(p.rec), n) = _Float16(r * kResidScale);
 }
 
-// Cell of this thread: block = one row segment of blockDim.x cells.
-__device__ __forceinline__ Cell locate(const Params& p) {
+// Block index in the order a step walks its tiles.  With 4- and 8-byte
+// storage the odd steps (ODD = t % 2) walk them backwards, so that a step
+// starts on the rows the previous step wrote last, which are still in the
+// 256 MB Infinity Cache (8192 x 2048: single phase FP32 stream-collide 210 ->
+// 191 us, free surface 70.0 -> 66.8 us).  The 16-bit formats measured no gain
+// on the free surface and 6 % less on the single phase, and keep one direction.
+template <class S, int ODD>
+__device__ __forceinline__ uint32_t walk_block() {
+  return (ODD && S::kBytes >= 4) ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+}
+
+// Cell of this thread: block b (in walk order) = one row segment of
+// blockDim.x cells.
+__device__ __forceinline__ Cell locate(const Params& p, uint32_t b = blockIdx.x) {
   uint32_t tx, y;
-  block_tile(blockIdx.x, p.tiles_x, p.ny, p.band_magic, p.last_magic, tx, y);
+  block_tile(b, p.tiles_x, p.ny, p.band_magic, p.last_magic, tx, y);
   const uint32_t x = tx * blockDim.x + threadIdx.x;
   return Cell{x, y, x + y * p.pitch, x < p.nx};
 }
 
 // Cell of this thread in a free-surface stream-collide launch over the window
 // of Params (block_tile_window).
-__device__ __forceinline__ Cell locate_window(const Params& p) {
+__device__ __forceinline__ Cell locate_window(const Params& p, uint32_t b = blockIdx.x) {
   uint32_t tx, y;
-  block_tile_window(p, blockIdx.x, tx, y);
+  block_tile_window(p, b, tx, y);
   const uint32_t x = tx * blockDim.x + threadIdx.x;
   return Cell{x, y, x + y * p.pitch, x < p.nx};
 }

@@ -4,7 +4,9 @@
 // FLB_PROBE_ESOTERIC moves every cell's nine populations through the
 // esoteric-pull slots exactly as stream-collide does (nine loads from the
 // cell's load slots, nine stores to its store slots, in place, parity
-// alternating per launch), with a multiply by 1 as "collision";
+// alternating per launch, and with 4-byte values the odd launches walking the
+// blocks backwards, as stream-collide does), with a multiply by 1 as
+// "collision";
 // FLB_PROBE_ESOTERIC16 does the same with 16-bit values (the FP16S / BF16S
 // pattern).  FLB_PROBE_COPY copies nine planes to nine other planes,
 // FLB_PROBE_INPLACE adds 1 to nine planes in place.  One thread per cell,
@@ -37,7 +39,10 @@ struct ProbeArgs {
 template <class T, int KIND, int ODD>
 __global__ void __launch_bounds__(kProbeBlock) k_probe(const ProbeArgs a) {
   uint32_t tx, y;
-  flb::block_tile(blockIdx.x, a.p.tiles_x, a.p.ny, a.p.band_magic, a.p.last_magic, tx, y);
+  // Block order as stream-collide's (walk_block): odd launches backwards with
+  // 4-byte values.
+  const uint32_t b = (ODD && sizeof(T) >= 4) ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+  flb::block_tile(b, a.p.tiles_x, a.p.ny, a.p.band_magic, a.p.last_magic, tx, y);
   const uint32_t x = tx * kProbeBlock + threadIdx.x;
   if (x >= a.p.nx) return;
   const uint64_t Np = a.p.Np;
