@@ -101,15 +101,15 @@ __device__ __forceinline__ void resid_store(const Params& p, uint32_t n, float r
   at(reinterpret_cast<_Float16*>(p.rec), n) = _Float16(r * kResidScale);
 }
 
-// Block index in the order a step walks its tiles.  With 4- and 8-byte
-// storage the odd steps (ODD = t % 2) walk them backwards, so that a step
-// starts on the rows the previous step wrote last, which are still in the
-// 256 MB Infinity Cache (8192 x 2048: single phase FP32 stream-collide 210 ->
-// 191 us, free surface 70.0 -> 66.8 us).  The 16-bit formats measured no gain
-// on the free surface and 6 % less on the single phase, and keep one direction.
+// Block index in the order a step walks its tiles: the odd steps (ODD = t % 2)
+// take the bands of the launch last to first, so that a step starts on the
+// rows the previous step wrote last, which are still in the 256 MB Infinity
+// Cache.  The launch is the window of Params (the whole grid: ncols = tiles_x,
+// win_magic = band_magic).
 template <class S, int ODD>
-__device__ __forceinline__ uint32_t walk_block() {
-  return (ODD && S::kBytes >= 4) ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+__device__ __forceinline__ uint32_t walk_block(const Params& p) {
+  if (!ODD) return blockIdx.x;
+  return band_reversed(blockIdx.x, p.walk_last, p.walk_bands, kBandRows * p.ncols, p.win_magic);
 }
 
 // Cell of this thread: block b (in walk order) = one row segment of

@@ -126,6 +126,7 @@ flb::Params params(const flb_handle* h, int bx) {
   p.band0 = 0;
   p.col0 = 0;
   p.ncols = p.tiles_x;
+  flb::set_walk(p, h->ny, p.tiles_x);
   p.band_magic = flb::div_magic(uint64_t(flb::kBandRows) * p.tiles_x);
   p.win_magic = p.band_magic;
   p.last_magic = flb::div_magic(flb::last_band_rows(h->ny));
@@ -654,6 +655,7 @@ void stream_collide_window(const flb_handle* h, flb::Params& p, int bx) {
   p.ncols = uint32_t(col1 - col0 + 1);
   p.win_magic = flb::div_magic(uint64_t(flb::kBandRows) * p.ncols);
   p.nblocks = uint32_t(int64_t(p.ncols) * rows);
+  flb::set_walk(p, uint32_t(rows), p.ncols);
 }
 
 // One lattice step on the handle's stream.  ev (optional) receives per-kernel

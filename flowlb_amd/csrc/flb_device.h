@@ -220,6 +220,9 @@ struct Params {
   // kBandRows rows from band0, block columns col0 .. col0 + ncols - 1.  The
   // whole grid is 0, 0, tiles_x.
   uint32_t band0, col0, ncols;
+  // Blocks in the last band of the launch and bands of the launch (the odd
+  // steps take the bands last to first: walk_block).
+  uint32_t walk_last, walk_bands;
   // div_magic(kBandRows * tiles_x), div_magic(kBandRows * ncols) and
   // div_magic(height of the last band): block_tile and block_tile_window
   // divide on the scalar unit (0 = divide instead).
@@ -322,6 +325,27 @@ __host__ __device__ __forceinline__ void block_tile(uint32_t b, uint32_t tiles_x
   y = b / tiles_x;
   tx = b - y * tiles_x;
   if (kTileOrder == 1) tx = (tx + y % tiles_x) % tiles_x;
+}
+
+// Forward-order index of block b when a launch takes its bands last to first,
+// each band's blocks in their forward order: `last` blocks in the last band,
+// `nbands` bands, `per` blocks in a full band (div_magic m).  The row
+// grouping of a band is kept: every band but the last is a whole number of
+// 32-block columns, so rows congruent mod 8 stay on one XCD.
+__host__ __device__ __forceinline__ uint32_t band_reversed(uint32_t b, uint32_t last,
+                                                           uint32_t nbands, uint32_t per,
+                                                           uint32_t m) {
+  if (b < last) return (nbands - 1u) * per + b;
+  const uint32_t b2 = b - last;
+  const uint32_t k = div_by(b2, per, m);
+  return (nbands - 2u - k) * per + (b2 - k * per);
+}
+
+// Params::walk_last / walk_bands of a launch over `rows` rows of `cols` block
+// columns.
+inline void set_walk(Params& p, uint32_t rows, uint32_t cols) {
+  p.walk_bands = (rows + kBandRows - 1) / kBandRows;
+  p.walk_last = (rows - (p.walk_bands - 1) * kBandRows) * cols;
 }
 
 // block_tile (band order) restricted to the window band0 / col0 / ncols of

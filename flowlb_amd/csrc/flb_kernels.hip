@@ -33,7 +33,7 @@ namespace flb {
 // and k_stream_collide_surface the others.
 template <class S, int ODD, bool FS, bool MOM, bool TRACE = false, bool DENSE = false>
 __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
-  const uint32_t b = walk_block<S, ODD>();
+  const uint32_t b = walk_block<S, ODD>(p);
   const Cell c = FS ? locate_window(p, b) : locate(p, b);
   uint32_t work = 0;
   if (FS) {
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(1024) k_stream_collide(const Params p) {
 template <class S, int ODD, bool MOM>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7)))
 k_stream_collide_all(const Params p) {
-  const Cell c = locate_window(p, walk_block<S, ODD>());
+  const Cell c = locate_window(p, walk_block<S, ODD>(p));
   const uint32_t work = chunk_work(p, c, true, true);
   if (!(work & (1u << kWorkSC)) || !c.valid) return;
   if (work & (1u << kWorkFS1))

@@ -4,9 +4,8 @@
 // FLB_PROBE_ESOTERIC moves every cell's nine populations through the
 // esoteric-pull slots exactly as stream-collide does (nine loads from the
 // cell's load slots, nine stores to its store slots, in place, parity
-// alternating per launch, and with 4-byte values the odd launches walking the
-// blocks backwards, as stream-collide does), with a multiply by 1 as
-// "collision";
+// alternating per launch, and the odd launches taking the bands last to first,
+// as stream-collide does), with a multiply by 1 as "collision";
 // FLB_PROBE_ESOTERIC16 does the same with 16-bit values (the FP16S / BF16S
 // pattern).  FLB_PROBE_COPY copies nine planes to nine other planes,
 // FLB_PROBE_INPLACE adds 1 to nine planes in place.  One thread per cell,
@@ -39,9 +38,11 @@ struct ProbeArgs {
 template <class T, int KIND, int ODD>
 __global__ void __launch_bounds__(kProbeBlock) k_probe(const ProbeArgs a) {
   uint32_t tx, y;
-  // Block order as stream-collide's (walk_block): odd launches backwards with
-  // 4-byte values.
-  const uint32_t b = (ODD && sizeof(T) >= 4) ? gridDim.x - 1u - blockIdx.x : blockIdx.x;
+  // Block order as stream-collide's (walk_block): odd launches take the bands
+  // last to first.
+  const uint32_t b = ODD ? flb::band_reversed(blockIdx.x, a.p.walk_last, a.p.walk_bands,
+                                              flb::kBandRows * a.p.tiles_x, a.p.band_magic)
+                         : blockIdx.x;
   flb::block_tile(b, a.p.tiles_x, a.p.ny, a.p.band_magic, a.p.last_magic, tx, y);
   const uint32_t x = tx * kProbeBlock + threadIdx.x;
   if (x >= a.p.nx) return;
@@ -103,6 +104,7 @@ extern "C" int flb_bandwidth_probe(int32_t device, uint32_t nx, uint32_t ny, int
   a.p.Np = flb::plane_stride(uint64_t(a.p.pitch) * ny, elem);
   a.p.band_magic = flb::div_magic(uint64_t(flb::kBandRows) * a.p.tiles_x);
   a.p.last_magic = flb::div_magic(flb::last_band_rows(ny));
+  flb::set_walk(a.p, ny, a.p.tiles_x);
   a.one = 1.0f;
   if ((uint64_t(a.p.pitch) + 128) * ny > 0xffffffffull / 2) return FLB_ERR_ARGUMENT;
   if (hipSetDevice(device) != hipSuccess) return FLB_ERR_DEVICE;
