@@ -1,7 +1,8 @@
 // Host checks of the index arithmetic of flb_device.h / flb_internal.h:
 // block_tile and block_tile_window with the scalar magic division give the
 // tiles of the plain division form for every block index of many grid shapes,
-// div_by equals n / d, and plane_stride keeps planes 256-byte aligned, at least
+// div_by equals n / d, band_reversed is a permutation in the order the odd
+// steps walk, and plane_stride keeps planes 256-byte aligned, at least
 // a plane long and 4 KiB past a multiple of 8 KiB apart.  Built and run by
 // tests/test_block_mapping.py.
 #include <cstdio>
@@ -65,6 +66,28 @@ int main() {
       seen[size_t(y) * t + tx]++;
     }
     for (int v : seen) if (v != 1) { if (bad++ < 5) printf("tile not visited once t=%u rows=%u\n", t, rows); break; }
+    ++checked;
+  }
+  // band_reversed (the odd steps' walk): a permutation of the launch's blocks,
+  // the last band first, then the full bands from the second-to-last down, each
+  // band's blocks in their forward order, and a block index congruent mod 8 to
+  // the others of its band's rows (the XCD grouping of a full band is kept).
+  for (uint32_t cols : {1u, 2u, 3u, 7u, 8u, 33u, 128u}) for (uint32_t rows : {1u, 5u, 31u, 32u, 33u, 64u, 100u, 2049u}) {
+    Params p{};
+    set_walk(p, rows, cols);
+    const uint32_t per = kBandRows * cols, m = div_magic(per), n = rows * cols;
+    std::vector<int> seen(n, 0);
+    for (uint32_t b = 0; b < n; ++b) {
+      const uint32_t f = band_reversed(b, p.walk_last, p.walk_bands, per, m);
+      if (f >= n) { if (bad++ < 5) printf("reversed out of range cols=%u rows=%u b=%u\n", cols, rows, b); continue; }
+      seen[f]++;
+      const uint32_t band = f / per;
+      const uint32_t want = b < p.walk_last ? p.walk_bands - 1 : p.walk_bands - 2 - (b - p.walk_last) / per;
+      const uint32_t off = b < p.walk_last ? b : (b - p.walk_last) % per;
+      if (band != want || f - band * per != off) { if (bad++ < 5) printf("reversed order cols=%u rows=%u b=%u\n", cols, rows, b); }
+      if (band + 1 < p.walk_bands && (b - p.walk_last) % 8 != (f - band * per) % 8) { if (bad++ < 5) printf("reversed xcd cols=%u rows=%u b=%u\n", cols, rows, b); }
+    }
+    for (int v : seen) if (v != 1) { if (bad++ < 5) printf("reversed not a permutation cols=%u rows=%u\n", cols, rows); break; }
     ++checked;
   }
   // div_by against n / d on large numerators
