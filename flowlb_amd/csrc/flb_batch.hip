@@ -37,7 +37,9 @@ struct Tile {
 template <class S, int ODD, bool MOM>
 __global__ void __launch_bounds__(1024) k_batch_step(const Tile* __restrict__ tiles, uint32_t ntiles,
                                                       const Params* __restrict__ inst) {
-  // Odd steps walk the list backwards with 4-byte storage (walk_block).
+  // With 4-byte storage the odd steps walk the list backwards, so that a step
+  // starts on the tiles the previous one wrote last, still in the Infinity
+  // Cache (+4 % on 256 culverts; 16-bit storage keeps one direction).
   const bool rev = ODD && S::kBytes >= 4;
   const auto at_pos = [&](uint32_t k) { return tiles[rev ? ntiles - 1u - k : k]; };
   uint32_t k = blockIdx.x;
@@ -73,7 +75,7 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(const Tile* __restric
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   for (uint32_t s = 0; s < nsteps; ++s) {
     const bool odd = ((odd0 + s) & 1u) != 0;
-    const bool rev = odd && S::kBytes >= 4;  // walk_block's order
+    const bool rev = odd && S::kBytes >= 4;  // k_batch_step's order
     for (uint32_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
       const Tile t = tiles[rev ? ntiles - 1u - k : k];
       const Params& p = inst[t.inst];
