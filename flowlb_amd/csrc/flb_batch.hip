@@ -13,11 +13,14 @@
 //   * persistent: one cooperative launch runs a chunk of steps, its resident
 //     blocks walking the tile list and meeting at a grid barrier between
 //     steps.
-// An autotuner times block sizes, grid sizes and both modes on the actual
-// instance set and keeps the fastest.
+// The odd steps take the tile list's bands last to first (a second copy of
+// the list), so that a step starts on what the Infinity Cache still holds of
+// the previous one.  An autotuner times block sizes, grid sizes and both modes
+// on the actual instance set and keeps the fastest.
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include <hip/hip_cooperative_groups.h>
@@ -37,11 +40,7 @@ struct Tile {
 template <class S, int ODD, bool MOM>
 __global__ void __launch_bounds__(1024) k_batch_step(const Tile* __restrict__ tiles, uint32_t ntiles,
                                                       const Params* __restrict__ inst) {
-  // With 4-byte storage the odd steps walk the list backwards, so that a step
-  // starts on the tiles the previous one wrote last, still in the Infinity
-  // Cache (+4 % on 256 culverts; 16-bit storage keeps one direction).
-  const bool rev = ODD && S::kBytes >= 4;
-  const auto at_pos = [&](uint32_t k) { return tiles[rev ? ntiles - 1u - k : k]; };
+  const auto at_pos = [&](uint32_t k) { return tiles[k]; };
   uint32_t k = blockIdx.x;
   if (k >= ntiles) return;
   Tile t = at_pos(k);
@@ -75,9 +74,9 @@ __global__ void __launch_bounds__(1024) k_batch_persistent(const Tile* __restric
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   for (uint32_t s = 0; s < nsteps; ++s) {
     const bool odd = ((odd0 + s) & 1u) != 0;
-    const bool rev = odd && S::kBytes >= 4;  // k_batch_step's order
+    const Tile* const list = tiles + (odd ? ntiles : 0u);  // the odd steps' order
     for (uint32_t k = blockIdx.x; k < ntiles; k += gridDim.x) {
-      const Tile t = tiles[rev ? ntiles - 1u - k : k];
+      const Tile t = list[k];
       const Params& p = inst[t.inst];
       const uint32_t x = t.tx * blockDim.x + threadIdx.x;
       if (x < p.nx) {
@@ -239,9 +238,24 @@ int build_tiles(flb_batch* b, int bx) {
                    b->owned.end());
     b->tiles = nullptr;
   }
-  int rc = balloc(b, &b->tiles, tl.size());
+  // The odd steps take the list's bands (runs of tiles of one instance and one
+  // 32-row band) last to first, each band's tiles in their order, so that a
+  // step starts on the tiles the previous one wrote last, which are still in
+  // the Infinity Cache.  Both orders go into one allocation, the odd one second.
+  std::vector<flb::Tile> both(tl);
+  both.reserve(2 * tl.size());
+  std::vector<std::pair<size_t, size_t>> runs;  // [begin, end) of each band run
+  for (size_t a = 0; a < tl.size();) {
+    size_t e = a + 1;
+    while (e < tl.size() && tl[e].inst == tl[a].inst && tl[e].row / 32 == tl[a].row / 32) ++e;
+    runs.emplace_back(a, e);
+    a = e;
+  }
+  for (size_t r = runs.size(); r-- > 0;)
+    both.insert(both.end(), tl.begin() + ptrdiff_t(runs[r].first), tl.begin() + ptrdiff_t(runs[r].second));
+  int rc = balloc(b, &b->tiles, both.size());
   if (rc) return rc;
-  BAT_HIP(b, hipMemcpy(b->tiles, tl.data(), tl.size() * sizeof(flb::Tile), hipMemcpyHostToDevice));
+  BAT_HIP(b, hipMemcpy(b->tiles, both.data(), both.size() * sizeof(flb::Tile), hipMemcpyHostToDevice));
   b->ntiles = uint32_t(tl.size());
   b->block_x = bx;
   return FLB_OK;
@@ -256,7 +270,7 @@ void drop_graph(flb_batch* b) {
 int launch_step(flb_batch* b, int odd, bool mom) {
   const uint32_t grid = b->grid > 0 ? uint32_t(b->grid) : b->ntiles;  // 0: a block per tile
   hipLaunchKernelGGL(flb::get_batch(b->storage, odd, mom), dim3(grid), dim3(b->block_x), 0,
-                     b->stream, b->tiles, b->ntiles, b->inst);
+                     b->stream, b->tiles + (odd ? b->ntiles : 0u), b->ntiles, b->inst);
   BAT_HIP(b, hipGetLastError());
   return FLB_OK;
 }
