@@ -37,19 +37,20 @@ struct Tile {
   uint32_t pad;
 };
 
+// One step over `tiles`, the list in this step's order (the odd steps get the
+// second copy).
 template <class S, int ODD, bool MOM>
 __global__ void __launch_bounds__(1024) k_batch_step(const Tile* __restrict__ tiles, uint32_t ntiles,
                                                       const Params* __restrict__ inst) {
-  const auto at_pos = [&](uint32_t k) { return tiles[k]; };
   uint32_t k = blockIdx.x;
   if (k >= ntiles) return;
-  Tile t = at_pos(k);
+  Tile t = tiles[k];
   // A grid-stride block loads its next tile's descriptor while it evaluates
   // this one (with a block per tile there is no next: the same line again).
   for (;;) {
     const uint32_t kn = k + gridDim.x;
     const bool more = kn < ntiles;
-    const Tile tn = at_pos(more ? kn : k);
+    const Tile tn = tiles[more ? kn : k];
     const Params& p = inst[t.inst];
     const uint32_t x = t.tx * blockDim.x + threadIdx.x;
     if (x < p.nx) {
